@@ -1,0 +1,153 @@
+// capi.cpp -- the extern "C" entry points declared in include/fluidsx.h.
+#include <new>
+#include <string>
+
+#include "fluidsx.h"
+#include "fx_common.hpp"
+#include "fx_scene.hpp"
+#include "particles.hpp"
+
+struct fx_system {
+  Particles p;
+};
+
+namespace fx {
+std::string& last_error() {
+  static thread_local std::string msg;
+  return msg;
+}
+}  // namespace fx
+
+#define FX_CHECK_SYS(s) \
+  if ((s) == nullptr) return fx::fail(FX_EINVAL, "%s: null fx_system", __func__)
+
+extern "C" {
+
+int fx_scene_derive(const fx_scene* scene, FParams_t* out) {
+  if (scene == nullptr || out == nullptr) return fx::fail(FX_EINVAL, "fx_scene_derive: null argument");
+  return fx::scene_derive(*scene, *out);
+}
+
+float fx_scene_spacing(const fx_scene* scene) {
+  return scene == nullptr ? 0.0f : fx::scene_spacing(*scene);
+}
+
+int fx_scene_fill(const fx_scene* scene, float* pos, uint32_t* clr, int32_t n) {
+  if (scene == nullptr || pos == nullptr) return fx::fail(FX_EINVAL, "fx_scene_fill: null argument");
+  return fx::scene_fill(*scene, pos, clr, n);
+}
+
+int fx_create(const char* path, fx_system** out) {
+  if (path == nullptr || out == nullptr) return fx::fail(FX_EINVAL, "fx_create: null argument");
+  *out = nullptr;
+  fx_system* s = new (std::nothrow) fx_system;
+  if (s == nullptr) return fx::fail(FX_ERANGE, "fx_create: out of host memory");
+  int st = s->p.Initialize(path);
+  if (st != FX_OK) {
+    delete s;
+    return st;
+  }
+  *out = s;
+  return FX_OK;
+}
+
+void fx_destroy(fx_system* s) { delete s; }
+
+int fx_restart(fx_system* s, const fx_scene* scene) {
+  FX_CHECK_SYS(s);
+  if (scene == nullptr) return fx::fail(FX_EINVAL, "fx_restart: null scene");
+  return s->p.Restart(*scene);
+}
+
+int fx_get_params(const fx_system* s, FParams_t* out) {
+  FX_CHECK_SYS(s);
+  if (out == nullptr) return fx::fail(FX_EINVAL, "fx_get_params: null argument");
+  *out = s->p.Params();
+  return FX_OK;
+}
+
+int fx_set_params(fx_system* s, const FParams_t* p) {
+  FX_CHECK_SYS(s);
+  if (p == nullptr) return fx::fail(FX_EINVAL, "fx_set_params: null argument");
+  return s->p.SetParams(*p);
+}
+
+int32_t fx_num_points(const fx_system* s) { return s == nullptr ? 0 : s->p.NumPoints(); }
+int32_t fx_grid_total(const fx_system* s) { return s == nullptr ? 0 : s->p.Params().gridTotal; }
+
+int32_t fx_buffer_stride(int32_t buf) {
+  switch (buf) {
+    case FPOS: case FVEL: case FVEVAL: case FFORCE: return 12;
+    case FCLR: case FPRESS: case FGCELL: case FGNDX: case FID: return 4;
+    default: return 0;
+  }
+}
+
+int fx_upload(fx_system* s, int32_t buf, const void* host, int64_t count) {
+  FX_CHECK_SYS(s);
+  return s->p.Upload(buf, host, count);
+}
+
+int fx_download(fx_system* s, int32_t buf, void* host, int64_t count) {
+  FX_CHECK_SYS(s);
+  return s->p.Download(buf, host, count);
+}
+
+int fx_download_accel(fx_system* s, int32_t buf, void* host, int64_t count) {
+  FX_CHECK_SYS(s);
+  return s->p.DownloadAccel(buf, host, count);
+}
+
+int fx_stage(fx_system* s, int32_t stage) {
+  FX_CHECK_SYS(s);
+  return s->p.Stage(stage);
+}
+
+int fx_run(fx_system* s, int32_t nsteps) {
+  FX_CHECK_SYS(s);
+  if (nsteps < 0) return fx::fail(FX_EINVAL, "fx_run: nsteps < 0");
+  for (int k = 0; k < nsteps; k++) FX_TRY(s->p.Run());
+  return FX_OK;
+}
+
+int fx_synchronize(fx_system* s) {
+  FX_CHECK_SYS(s);
+  return s->p.Synchronize();
+}
+
+int fx_scan(fx_system* s, const uint32_t* in, uint32_t* out, int32_t n) {
+  FX_CHECK_SYS(s);
+  return s->p.Scan(in, out, n);
+}
+
+int fx_graph_capture(fx_system* s) {
+  FX_CHECK_SYS(s);
+  return s->p.CaptureGraph();
+}
+
+int fx_graph_launch(fx_system* s, int32_t nlaunch) {
+  FX_CHECK_SYS(s);
+  if (nlaunch < 0) return fx::fail(FX_EINVAL, "fx_graph_launch: nlaunch < 0");
+  return s->p.LaunchGraph(nlaunch);
+}
+
+int fx_profile_stages(fx_system* s, int32_t nsteps, double* ms_out) {
+  FX_CHECK_SYS(s);
+  return s->p.ProfileStages(nsteps, ms_out);
+}
+
+int fx_time_steps(fx_system* s, int32_t nsteps, int32_t use_graph, double* ms_total) {
+  FX_CHECK_SYS(s);
+  return s->p.TimeSteps(nsteps, use_graph != 0, ms_total);
+}
+
+int fx_set_block_size(fx_system* s, int32_t threads) {
+  FX_CHECK_SYS(s);
+  return s->p.SetBlockSize(threads);
+}
+
+const char* fx_last_error(void) { return fx::last_error().c_str(); }
+
+const char* fx_version(void) { return "fluidsx 0.1 gfx950"; }
+
+}  // extern "C"

@@ -1,0 +1,498 @@
+// particles.cpp -- host side of the SPH step (behaviour of
+// fluids5.0/src/particles.cpp: Initialize :68-93, Restart :95-119,
+// ReallocateParticles :267-300, Run :382-394, RebuildAccelGrid :548-626,
+// UpdateParams :657-679, stage methods :716-919).
+#include "particles.hpp"
+
+#include <cstring>
+#include <vector>
+
+#include "fx_common.hpp"
+#include "fx_scene.hpp"
+
+#define SCAN_BLOCKSIZE 512
+#define SCAN_ELEMS (2 * SCAN_BLOCKSIZE)
+
+static const char* kKernelNames[FUNC_MAX] = {
+    "insertParticles",   "countingSortFull", "computePressure", "computeForce",
+    "advanceParticles",  "prefixSum",        "prefixFixup",     "countingSortIndex",
+    "fxFillUint",        "advanceParticlesClock", "fxClockTick"};
+
+Particles::Particles() { memset(&m_Params, 0, sizeof(m_Params)); }
+
+Particles::~Particles() {
+  ReleaseGraph();
+  m_Points.DeleteAllBuffers();
+  m_PointsTemp.DeleteAllBuffers();
+  m_Accel.DeleteAllBuffers();
+  if (m_Tables) (void)hipFree(m_Tables);
+  if (m_Clock) (void)hipFree(m_Clock);
+  if (m_Stream) (void)hipStreamDestroy(m_Stream);
+  if (m_Module) (void)hipModuleUnload(m_Module);
+}
+
+int Particles::Initialize(const char* path) {
+  if (m_Module != nullptr) return fx::fail(FX_ESTATE, "Initialize: already initialized");
+  FX_HIP(hipModuleLoad(&m_Module, path), "Initialize");
+  for (int f = 0; f < FUNC_MAX; f++)
+    FX_HIP(hipModuleGetFunction(&m_Func[f], m_Module, kKernelNames[f]), kKernelNames[f]);
+  FX_TRY(m_Points.AssignToGPU("FPnts", m_Module));
+  FX_TRY(m_PointsTemp.AssignToGPU("FPntTmp", m_Module));
+  FX_TRY(m_Accel.AssignToGPU("FAccel", m_Module));
+  size_t len = 0;
+  FX_HIP(hipModuleGetGlobal(&m_cuParams, &len, m_Module, "FParams"), "Initialize");
+  if (len != sizeof(FParams_t))
+    return fx::fail(FX_EMISMATCH, "Initialize: FParams has %zu bytes, FParams_t has %zu", len,
+                    sizeof(FParams_t));
+  FX_HIP(hipStreamCreate(&m_Stream), "Initialize");
+  FX_HIP(hipMalloc(&m_Tables, 2 * sizeof(cuDataX)), "Initialize");
+  FX_HIP(hipMalloc((void**)&m_Clock, sizeof(float)), "Initialize");
+  m_Params.numThreads = 256;
+  return FX_OK;
+}
+
+int Particles::Launch(int fid, unsigned blocks, unsigned threads, void** args) {
+  if (blocks == 0) return FX_OK;
+  FX_HIP(hipModuleLaunchKernel(m_Func[fid], blocks, 1, 1, threads, 1, 1, 0, m_Stream, args, nullptr),
+         kKernelNames[fid]);
+  return FX_OK;
+}
+
+int Particles::Restart(const fx_scene& sc) {
+  if (m_Module == nullptr) return fx::fail(FX_ESTATE, "Restart: call Initialize first");
+  FX_HIP(hipStreamSynchronize(m_Stream), "Restart");
+  ReleaseGraph();
+  const int threads = m_Params.numThreads > 0 ? m_Params.numThreads : 256;
+  FParams_t P;
+  FX_TRY(fx::scene_derive(sc, P));
+  m_Params = P;
+  mNumPoints = 0;
+  mMaxPoints = sc.num_points;
+  FX_TRY(ReallocateParticles(mMaxPoints));
+  fx::set_geometry(m_Params, threads);
+  FX_TRY(RebuildAccelGrid());
+
+  // AddPointsInVolume: lattice into the host copies, other fields zero
+  const int n = fx::scene_fill(sc, reinterpret_cast<float*>(m_Points.bufF3(FPOS)), m_Points.bufUI(FCLR),
+                               mMaxPoints);
+  if (n < 0) return n;
+  if (n != mMaxPoints)
+    return fx::fail(FX_ERANGE, "Restart: lattice holds %d of %d particles", n, mMaxPoints);
+  memset(m_Points.bufF3(FVEL), 0, sizeof(fx_float3) * (size_t)n);
+  memset(m_Points.bufF3(FVEVAL), 0, sizeof(fx_float3) * (size_t)n);
+  memset(m_Points.bufF3(FFORCE), 0, sizeof(fx_float3) * (size_t)n);
+  memset(m_Points.bufF(FPRESS), 0, sizeof(float) * (size_t)n);
+  memset(m_Points.bufI(FGCELL), 0, sizeof(int) * (size_t)n);
+  memset(m_Points.bufI(FGNDX), 0, sizeof(int) * (size_t)n);
+  for (int i = 0; i < n; i++) m_Points.bufUI(FID)[i] = (unsigned)i;
+  mNumPoints = n;
+  m_Points.SetNum(n);
+  m_PointsTemp.SetNum(n);
+
+  FX_TRY(UpdateParams());
+  FX_TRY(CommitAll());
+  m_Frame = 0;
+  return FX_OK;
+}
+
+int Particles::ReallocateParticles(int maxpnt) {
+  // particles.cpp:267-300, plus the FID slot; GL interop flags become device memory
+  mMaxPoints = maxpnt;
+  m_Params.pnum = maxpnt;
+  m_Points.DeleteAllBuffers();
+  const uchar fl = DT_CPU | DT_CUMEM;
+  FX_TRY(m_Points.AddBuffer(FPOS, "pos", sizeof(fx_float3), maxpnt, fl));
+  FX_TRY(m_Points.AddBuffer(FCLR, "clr", sizeof(unsigned), maxpnt, fl));
+  FX_TRY(m_Points.AddBuffer(FVEL, "vel", sizeof(fx_float3), maxpnt, fl));
+  FX_TRY(m_Points.AddBuffer(FVEVAL, "veval", sizeof(fx_float3), maxpnt, fl));
+  FX_TRY(m_Points.AddBuffer(FPRESS, "press", sizeof(float), maxpnt, fl));
+  FX_TRY(m_Points.AddBuffer(FFORCE, "force", sizeof(fx_float3), maxpnt, fl));
+  FX_TRY(m_Points.AddBuffer(FGCELL, "gcell", sizeof(int), maxpnt, fl));
+  FX_TRY(m_Points.AddBuffer(FGNDX, "gndx", sizeof(int), maxpnt, fl));
+  FX_TRY(m_Points.AddBuffer(FID, "id", sizeof(unsigned), maxpnt, fl));
+  m_Points.SetBufferUsage(FPOS, DT_FLOAT3);
+  m_Points.SetBufferUsage(FCLR, DT_UINT);
+  m_Points.SetBufferUsage(FVEL, DT_FLOAT3);
+  m_Points.SetBufferUsage(FVEVAL, DT_FLOAT3);
+  m_Points.SetBufferUsage(FPRESS, DT_FLOAT);
+  m_Points.SetBufferUsage(FFORCE, DT_FLOAT3);
+  m_Points.SetBufferUsage(FGCELL, DT_INT);
+  m_Points.SetBufferUsage(FGNDX, DT_INT);
+  m_Points.SetBufferUsage(FID, DT_UINT);
+  FX_TRY(m_PointsTemp.MatchAllBuffers(&m_Points, DT_CUMEM));
+  m_Parity = 0;
+  return FX_OK;
+}
+
+int Particles::RebuildAccelGrid() {
+  // particles.cpp:548-626; the counts and offsets carry one extra slot
+  // (index gridTotal) for out-of-range particles.
+  FX_TRY(fx::rebuild_grid(m_Params));
+  const int numElem1 = m_Params.gridTotal + 1;
+  const int numElem2 = numElem1 / SCAN_ELEMS + 1;
+  const int numElem3 = numElem2 / SCAN_ELEMS + 1;
+  if (numElem3 > SCAN_ELEMS) return fx::fail(FX_ERANGE, "RebuildAccelGrid: %d cells exceed the scan", numElem1);
+  m_Accel.DeleteAllBuffers();
+  FX_TRY(m_Accel.AddBuffer(AGRID, "grid", sizeof(unsigned), mMaxPoints, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(AGRIDCNT, "gridcnt", sizeof(unsigned), numElem1, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(AGRIDOFF, "gridoff", sizeof(unsigned), numElem1, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(AAUXARRAY1, "aux1", sizeof(unsigned), numElem2, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(AAUXSCAN1, "scan1", sizeof(unsigned), numElem2, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(AAUXARRAY2, "aux2", sizeof(unsigned), numElem3, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(AAUXSCAN2, "scan2", sizeof(unsigned), numElem3, DT_CUMEM));
+  for (int b = 0; b <= AAUXSCAN2; b++) m_Accel.SetBufferUsage(b, DT_UINT);
+  FX_TRY(UpdateGPUAccess());
+  return FX_OK;
+}
+
+int Particles::UpdateGPUAccess() {
+  // particles.cpp:649-655, plus the device copy of both sets' tables used by
+  // the per-step exchange
+  cuDataX t[2];
+  m_Points.FillTable(&t[m_Parity]);
+  m_PointsTemp.FillTable(&t[m_Parity ^ 1]);
+  FX_HIP(hipMemcpyHtoD(m_Tables, t, sizeof(t)), "UpdateGPUAccess");
+  FX_TRY(m_Points.UpdateGPUAccess());
+  FX_TRY(m_PointsTemp.UpdateGPUAccess());
+  FX_TRY(m_Accel.UpdateGPUAccess());
+  return FX_OK;
+}
+
+int Particles::UpdateParams() {
+  fx::update_params(m_Params);
+  FX_HIP(hipStreamSynchronize(m_Stream), "UpdateParams");
+  FX_HIP(hipMemcpyHtoD(m_cuParams, &m_Params, sizeof(FParams_t)), "UpdateParams");
+  return FX_OK;
+}
+
+int Particles::SetParams(const FParams_t& p) {
+  if (mMaxPoints == 0) return fx::fail(FX_ESTATE, "SetParams: call Restart first");
+  const FParams_t& q = m_Params;
+  if (p.gridRes.x != q.gridRes.x || p.gridRes.y != q.gridRes.y || p.gridRes.z != q.gridRes.z ||
+      memcmp(&p.gridMin, &q.gridMin, sizeof(fx_float3)) != 0 ||
+      memcmp(&p.gridDelta, &q.gridDelta, sizeof(fx_float3)) != 0 || p.gridSrch != q.gridSrch ||
+      p.pnum != q.pnum || p.psmoothradius != q.psmoothradius || p.sim_scale != q.sim_scale)
+    return fx::fail(FX_ESTATE, "SetParams: the grid, smoothing radius, scale and count are fixed by Restart");
+  ReleaseGraph();
+  const int threads = m_Params.numThreads;
+  m_Params = p;
+  fx::set_geometry(m_Params, threads);
+  return UpdateParams();
+}
+
+int Particles::SetBlockSize(int threads) {
+  if (threads < 64 || threads > 1024 || threads % 64 != 0)
+    return fx::fail(FX_EINVAL, "SetBlockSize: %d is not a multiple of 64 in [64, 1024]", threads);
+  ReleaseGraph();
+  m_Params.numThreads = threads;
+  if (mMaxPoints > 0) fx::set_geometry(m_Params, threads);
+  return FX_OK;
+}
+
+int Particles::CommitAll() {
+  FX_HIP(hipStreamSynchronize(m_Stream), "CommitAll");
+  return m_Points.CommitAll(m_Stream);
+}
+
+// ------------------------------------------------------------------ stages
+
+int Particles::InsertParticles() {
+  // particles.cpp:716-730.  Only the counts need clearing: gcell and gndx of
+  // every particle are written by the kernel.
+  int ncell = m_Params.gridTotal + 1;
+  unsigned zero = 0;
+  void* cnt = m_Accel.gpu(AGRIDCNT);
+  void* fargs[] = {&cnt, &zero, &ncell};
+  FX_TRY(Launch(FUNC_FILL, (unsigned)((ncell + 255) / 256), 256, fargs));
+  void* args[] = {&mNumPoints};
+  return Launch(FUNC_INSERT, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args);
+}
+
+int Particles::PrefixScanParticles() {
+  // particles.cpp:774-815, over gridTotal+1 counts
+  return ScanLevels(m_Accel.gpu(AGRIDCNT), m_Accel.gpu(AGRIDOFF), m_Accel.gpu(AAUXARRAY1),
+                    m_Accel.gpu(AAUXSCAN1), m_Accel.gpu(AAUXARRAY2), m_Accel.gpu(AAUXSCAN2),
+                    m_Params.gridTotal + 1);
+}
+
+int Particles::ScanLevels(void* array1, void* scan1, void* array2, void* scan2, void* array3, void* scan3,
+                          int n) {
+  // exclusive scan in up to three levels of 1024-element blocks
+  int numElem1 = n;
+  int numElem2 = numElem1 / SCAN_ELEMS + 1;
+  int numElem3 = numElem2 / SCAN_ELEMS + 1;
+  if (numElem3 > SCAN_ELEMS) return fx::fail(FX_ERANGE, "scan: %d elements exceed 1024^3", n);
+  int zon = 1;
+  void* argsA[] = {&array1, &scan1, &array2, &numElem1, &zon};
+  FX_TRY(Launch(FUNC_FPREFIXSUM, (unsigned)numElem2, SCAN_BLOCKSIZE, argsA));
+  void* argsB[] = {&array2, &scan2, &array3, &numElem2, &zon};
+  FX_TRY(Launch(FUNC_FPREFIXSUM, (unsigned)numElem3, SCAN_BLOCKSIZE, argsB));
+  if (numElem3 > 1) {
+    void* nptr = nullptr;
+    void* argsC[] = {&array3, &scan3, &nptr, &numElem3, &zon};
+    FX_TRY(Launch(FUNC_FPREFIXSUM, 1, SCAN_BLOCKSIZE, argsC));
+    void* argsD[] = {&scan2, &scan3, &numElem2};
+    FX_TRY(Launch(FUNC_FPREFIXFIXUP, (unsigned)numElem3, SCAN_BLOCKSIZE, argsD));
+  }
+  void* argsE[] = {&scan1, &scan2, &numElem1};
+  return Launch(FUNC_FPREFIXFIXUP, (unsigned)numElem2, SCAN_BLOCKSIZE, argsE);
+}
+
+int Particles::PublishTables() {
+  // FPnts <- table of the set m_Points now holds, FPntTmp <- the other one
+  char* tab = static_cast<char*>(m_Tables);
+  FX_HIP(hipMemcpyAsync(m_Points.SymbolAddress(), tab + m_Parity * sizeof(cuDataX), sizeof(cuDataX),
+                        hipMemcpyDeviceToDevice, m_Stream), "PublishTables");
+  FX_HIP(hipMemcpyAsync(m_PointsTemp.SymbolAddress(), tab + (m_Parity ^ 1) * sizeof(cuDataX),
+                        sizeof(cuDataX), hipMemcpyDeviceToDevice, m_Stream), "PublishTables");
+  return FX_OK;
+}
+
+int Particles::CountingSort() {
+  // particles.cpp:832-842.  CopyAllToTemp is replaced by exchanging the two
+  // sets: FPntTmp then holds the pre-sort data, FPnts receives the sort.
+  FX_TRY(m_Points.SwapGPU(&m_PointsTemp));
+  m_Parity ^= 1;
+  FX_TRY(PublishTables());
+  void* args[] = {&mNumPoints};
+  FX_TRY(Launch(FUNC_SORT_INDEX, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args));
+  return Launch(FUNC_COUNTING_SORT, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args);
+}
+
+int Particles::ComputePressure() {
+  void* args[] = {&mNumPoints};
+  return Launch(FUNC_COMPUTE_PRESS, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args);
+}
+
+int Particles::ComputeForce() {
+  void* args[] = {&mNumPoints};
+  return Launch(FUNC_COMPUTE_FORCE, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args);
+}
+
+int Particles::Advance() {
+  // particles.cpp:413-418 (launched over pnum, which equals mNumPoints here)
+  float dt = m_Params.dt, ss = m_Params.sim_scale;
+  int n = m_Params.pnum;
+  if (m_Capturing) {
+    void* args[] = {&m_Clock, &dt, &ss, &n};
+    FX_TRY(Launch(FUNC_ADVANCE_CLOCK, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args));
+    void* targs[] = {&m_Clock, &dt};
+    return Launch(FUNC_CLOCK_TICK, 1, 64, targs);
+  }
+  float time = m_Params.time;
+  void* args[] = {&time, &dt, &ss, &n};
+  return Launch(FUNC_ADVANCE, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args);
+}
+
+void Particles::AdvanceTime() {
+  m_Params.time += m_Params.dt;
+  m_Frame++;
+}
+
+int Particles::Stage(int stage) {
+  if (mNumPoints == 0) return fx::fail(FX_ESTATE, "stage: call Restart first");
+  switch (stage) {
+    case FX_STAGE_INSERT: return InsertParticles();
+    case FX_STAGE_PREFIX: return PrefixScanParticles();
+    case FX_STAGE_SORT: return CountingSort();
+    case FX_STAGE_PRESSURE: return ComputePressure();
+    case FX_STAGE_FORCE: return ComputeForce();
+    case FX_STAGE_ADVANCE: {
+      FX_TRY(Advance());
+      AdvanceTime();
+      return FX_OK;
+    }
+    default: return fx::fail(FX_EINVAL, "stage: unknown stage %d", stage);
+  }
+}
+
+int Particles::Run() {
+  if (mNumPoints == 0) return fx::fail(FX_ESTATE, "Run: call Restart first");
+  FX_TRY(InsertParticles());
+  FX_TRY(PrefixScanParticles());
+  FX_TRY(CountingSort());
+  FX_TRY(ComputePressure());
+  FX_TRY(ComputeForce());
+  FX_TRY(Advance());
+  AdvanceTime();
+  return FX_OK;
+}
+
+// ------------------------------------------------------------------ graphs
+
+void Particles::ReleaseGraph() {
+  if (m_GraphExec) (void)hipGraphExecDestroy(m_GraphExec);
+  if (m_Graph) (void)hipGraphDestroy(m_Graph);
+  m_GraphExec = nullptr;
+  m_Graph = nullptr;
+}
+
+int Particles::CaptureGraph() {
+  if (mNumPoints == 0) return fx::fail(FX_ESTATE, "CaptureGraph: call Restart first");
+  FX_HIP(hipStreamSynchronize(m_Stream), "CaptureGraph");
+  ReleaseGraph();
+  FX_HIP(hipStreamBeginCapture(m_Stream, hipStreamCaptureModeThreadLocal), "CaptureGraph");
+  m_Capturing = true;
+  int st = FX_OK;
+  for (int s = 0; s < 2 && st == FX_OK; s++) {
+    st = InsertParticles();
+    if (st == FX_OK) st = PrefixScanParticles();
+    if (st == FX_OK) st = CountingSort();
+    if (st == FX_OK) st = ComputePressure();
+    if (st == FX_OK) st = ComputeForce();
+    if (st == FX_OK) st = Advance();
+  }
+  m_Capturing = false;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(m_Stream, &g);
+  if (st != FX_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return fx::hip_fail(e, "CaptureGraph", "hipStreamEndCapture");
+  m_Graph = g;
+  m_GraphParity = m_Parity;
+  FX_HIP(hipGraphInstantiate(&m_GraphExec, m_Graph, nullptr, nullptr, 0), "CaptureGraph");
+  return FX_OK;
+}
+
+int Particles::LaunchGraph(int nlaunch) {
+  // the graph replays the set exchange it was captured with
+  if (m_GraphExec == nullptr || m_GraphParity != m_Parity) FX_TRY(CaptureGraph());
+  // the device clock starts from the host time of the first replayed step
+  FX_HIP(hipStreamSynchronize(m_Stream), "LaunchGraph");
+  FX_HIP(hipMemcpyHtoD(m_Clock, &m_Params.time, sizeof(float)), "LaunchGraph");
+  for (int k = 0; k < nlaunch; k++) {
+    FX_HIP(hipGraphLaunch(m_GraphExec, m_Stream), "LaunchGraph");
+    AdvanceTime();
+    AdvanceTime();
+  }
+  return FX_OK;
+}
+
+// ---------------------------------------------------------------- transfers
+
+int Particles::Upload(int buf, const void* host, int64_t count) {
+  DataPtr* b = m_Points.GetBuffer(buf);
+  if (b == nullptr || host == nullptr) return fx::fail(FX_EINVAL, "upload: unknown buffer %d", buf);
+  if (count < 0 || (uint64_t)count > b->mMax) return fx::fail(FX_ERANGE, "upload: %lld elements exceed %llu",
+                                                             (long long)count, (unsigned long long)b->mMax);
+  FX_HIP(hipStreamSynchronize(m_Stream), "upload");
+  FX_HIP(hipMemcpyHtoD(b->mGpu, const_cast<void*>(host), (size_t)count * b->mStride), "upload");
+  return FX_OK;
+}
+
+int Particles::Download(int buf, void* host, int64_t count) {
+  DataPtr* b = m_Points.GetBuffer(buf);
+  if (b == nullptr || host == nullptr) return fx::fail(FX_EINVAL, "download: unknown buffer %d", buf);
+  if (count < 0 || (uint64_t)count > b->mMax) return fx::fail(FX_ERANGE, "download: %lld elements exceed %llu",
+                                                             (long long)count, (unsigned long long)b->mMax);
+  FX_HIP(hipStreamSynchronize(m_Stream), "download");
+  FX_HIP(hipMemcpyDtoH(host, b->mGpu, (size_t)count * b->mStride), "download");
+  return FX_OK;
+}
+
+int Particles::DownloadAccel(int buf, void* host, int64_t count) {
+  DataPtr* b = m_Accel.GetBuffer(buf);
+  if (b == nullptr || host == nullptr) return fx::fail(FX_EINVAL, "download_accel: unknown buffer %d", buf);
+  if (count < 0 || (uint64_t)count > b->mMax)
+    return fx::fail(FX_ERANGE, "download_accel: %lld elements exceed %llu", (long long)count,
+                    (unsigned long long)b->mMax);
+  FX_HIP(hipStreamSynchronize(m_Stream), "download_accel");
+  FX_HIP(hipMemcpyDtoH(host, b->mGpu, (size_t)count * b->mStride), "download_accel");
+  return FX_OK;
+}
+
+int Particles::Scan(const uint32_t* in, uint32_t* out, int n) {
+  // The role of the reference's TestPrefixSum (fluids4.0/fluid_system.cpp:2120-2160):
+  // the grid scan applied to caller data, through temporary device buffers.
+  if (m_Module == nullptr) return fx::fail(FX_ESTATE, "scan: call Initialize first");
+  if (n <= 0 || in == nullptr || out == nullptr) return fx::fail(FX_EINVAL, "scan: bad arguments");
+  const int n2 = n / SCAN_ELEMS + 1, n3 = n2 / SCAN_ELEMS + 1;
+  if (n3 > SCAN_ELEMS) return fx::fail(FX_ERANGE, "scan: %d elements exceed 1024^3", n);
+  void* b[6] = {};
+  const size_t sz[6] = {(size_t)n, (size_t)n, (size_t)n2, (size_t)n2, (size_t)n3, (size_t)n3};
+  int st = FX_OK;
+  for (int k = 0; k < 6 && st == FX_OK; k++)
+    if (hipMalloc(&b[k], sz[k] * sizeof(uint32_t)) != hipSuccess) st = fx::fail(FX_EHIP, "scan: hipMalloc");
+  if (st == FX_OK && hipMemcpyHtoD(b[0], const_cast<uint32_t*>(in), (size_t)n * 4) != hipSuccess)
+    st = fx::fail(FX_EHIP, "scan: upload");
+  if (st == FX_OK) st = ScanLevels(b[0], b[1], b[2], b[3], b[4], b[5], n);
+  if (st == FX_OK && hipStreamSynchronize(m_Stream) != hipSuccess) st = fx::fail(FX_EHIP, "scan: run");
+  if (st == FX_OK && hipMemcpyDtoH(out, b[1], (size_t)n * 4) != hipSuccess) st = fx::fail(FX_EHIP, "scan: download");
+  for (auto p : b)
+    if (p) (void)hipFree(p);
+  return st;
+}
+
+int Particles::Synchronize() {
+  FX_HIP(hipStreamSynchronize(m_Stream), "synchronize");
+  return FX_OK;
+}
+
+// ------------------------------------------------------------------- timing
+
+int Particles::ProfileStages(int nsteps, double* ms_out) {
+  if (nsteps <= 0 || ms_out == nullptr) return fx::fail(FX_EINVAL, "profile_stages: bad arguments");
+  hipEvent_t ev[FX_NUM_STAGES + 1];
+  for (auto& e : ev) FX_HIP(hipEventCreate(&e), "profile_stages");
+  for (int k = 0; k <= FX_NUM_STAGES; k++) ms_out[k] = 0.0;
+  int st = FX_OK;
+  for (int s = 0; s < nsteps && st == FX_OK; s++) {
+    for (int k = 0; k < FX_NUM_STAGES && st == FX_OK; k++) {
+      if (hipEventRecord(ev[k], m_Stream) != hipSuccess) st = fx::fail(FX_EHIP, "profile_stages: record");
+      if (st == FX_OK) st = Stage(k);
+    }
+    if (st == FX_OK && hipEventRecord(ev[FX_NUM_STAGES], m_Stream) != hipSuccess)
+      st = fx::fail(FX_EHIP, "profile_stages: record");
+    if (st == FX_OK && hipEventSynchronize(ev[FX_NUM_STAGES]) != hipSuccess)
+      st = fx::fail(FX_EHIP, "profile_stages: synchronize");
+    for (int k = 0; k < FX_NUM_STAGES && st == FX_OK; k++) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) st = fx::fail(FX_EHIP, "profile_stages: elapsed");
+      ms_out[k] += ms;
+    }
+    float tot = 0.0f;
+    if (st == FX_OK && hipEventElapsedTime(&tot, ev[0], ev[FX_NUM_STAGES]) == hipSuccess)
+      ms_out[FX_NUM_STAGES] += tot;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (st != FX_OK) return st;
+  for (int k = 0; k <= FX_NUM_STAGES; k++) ms_out[k] /= nsteps;
+  return FX_OK;
+}
+
+int Particles::TimeSteps(int nsteps, bool use_graph, double* ms_total) {
+  if (nsteps < 0 || ms_total == nullptr) return fx::fail(FX_EINVAL, "time_steps: bad arguments");
+  if (use_graph && (m_GraphExec == nullptr || m_GraphParity != m_Parity)) FX_TRY(CaptureGraph());
+  if (use_graph) {
+    // the device clock must hold the host time before the timed region
+    FX_HIP(hipStreamSynchronize(m_Stream), "time_steps");
+    FX_HIP(hipMemcpyHtoD(m_Clock, &m_Params.time, sizeof(float)), "time_steps");
+  }
+  hipEvent_t a, b;
+  FX_HIP(hipEventCreate(&a), "time_steps");
+  FX_HIP(hipEventCreate(&b), "time_steps");
+  int st = FX_OK;
+  if (hipEventRecord(a, m_Stream) != hipSuccess) st = fx::fail(FX_EHIP, "time_steps: record");
+  if (st == FX_OK) {
+    if (use_graph) {
+      for (int k = 0; k < nsteps / 2 && st == FX_OK; k++) {
+        if (hipGraphLaunch(m_GraphExec, m_Stream) != hipSuccess) st = fx::fail(FX_EHIP, "time_steps: graph launch");
+        AdvanceTime();
+        AdvanceTime();
+      }
+      if (st == FX_OK && (nsteps & 1)) st = Run();
+    } else {
+      for (int k = 0; k < nsteps && st == FX_OK; k++) st = Run();
+    }
+  }
+  if (st == FX_OK && hipEventRecord(b, m_Stream) != hipSuccess) st = fx::fail(FX_EHIP, "time_steps: record");
+  if (st == FX_OK && hipEventSynchronize(b) != hipSuccess) st = fx::fail(FX_EHIP, "time_steps: synchronize");
+  float ms = 0.0f;
+  if (st == FX_OK && hipEventElapsedTime(&ms, a, b) != hipSuccess) st = fx::fail(FX_EHIP, "time_steps: elapsed");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms_total = ms;
+  return st;
+}

@@ -1,0 +1,110 @@
+// particles.hpp -- host orchestration of the SPH step on one MI355X.
+//
+// Mirrors the reference's Particles class (fluids5.0/src/particles.h:29-96):
+// same stage methods in the same order, same DataX sets (m_Points,
+// m_PointsTemp, m_Accel), same FParams_t block, kernels and globals resolved
+// by name from one code object.  Differences, all on the device-memory side:
+//   - CopyAllToTemp's eight device copies (particles.cpp:698-701) become an
+//     exchange of the two sets' device pointers plus a 2 x 128-byte table
+//     update, so the counting sort still reads FPntTmp and writes FPnts;
+//   - all work goes to one owned HIP stream, which can be captured into a
+//     graph;
+//   - errors are status codes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "datax.hpp"
+#include "fluidsx.h"
+
+// kernel slots (particles.h:104-111 plus the additions of this module)
+#define FUNC_INSERT 0
+#define FUNC_COUNTING_SORT 1
+#define FUNC_COMPUTE_PRESS 2
+#define FUNC_COMPUTE_FORCE 3
+#define FUNC_ADVANCE 4
+#define FUNC_FPREFIXSUM 5
+#define FUNC_FPREFIXFIXUP 6
+#define FUNC_SORT_INDEX 7
+#define FUNC_FILL 8
+#define FUNC_ADVANCE_CLOCK 9
+#define FUNC_CLOCK_TICK 10
+#define FUNC_MAX 11
+
+class Particles {
+ public:
+  Particles();
+  ~Particles();
+  Particles(const Particles&) = delete;
+  Particles& operator=(const Particles&) = delete;
+
+  // set-up (particles.cpp:68-119, 267-300, 548-679)
+  int Initialize(const char* code_object_path);
+  int Restart(const fx_scene& scene);
+  int ReallocateParticles(int maxpnt);
+  int RebuildAccelGrid();
+  int UpdateParams();
+  int UpdateGPUAccess();
+  int SetParams(const FParams_t& p);
+  int SetBlockSize(int threads);
+
+  // the step (particles.cpp:382-400)
+  int Run();
+  int InsertParticles();
+  int PrefixScanParticles();
+  int CountingSort();
+  int ComputePressure();
+  int ComputeForce();
+  int Advance();
+  void AdvanceTime();
+  int Stage(int stage);
+
+  // graphs
+  int CaptureGraph();
+  int LaunchGraph(int nlaunch);
+
+  // transfers (particles.cpp:703-711)
+  int CommitAll();
+  int Upload(int buf, const void* host, int64_t count);
+  int Download(int buf, void* host, int64_t count);
+  int DownloadAccel(int buf, void* host, int64_t count);
+  int Synchronize();
+  int Scan(const uint32_t* in, uint32_t* out, int n);
+
+  // timing
+  int ProfileStages(int nsteps, double* ms_out);
+  int TimeSteps(int nsteps, bool use_graph, double* ms_total);
+
+  int NumPoints() const { return mNumPoints; }
+  const FParams_t& Params() const { return m_Params; }
+  hipStream_t Stream() const { return m_Stream; }
+
+ private:
+  int Launch(int fid, unsigned blocks, unsigned threads, void** args);
+  int ScanLevels(void* array1, void* scan1, void* array2, void* scan2, void* array3, void* scan3, int n);
+  int PublishTables();
+  void ReleaseGraph();
+
+  hipModule_t m_Module = nullptr;
+  hipFunction_t m_Func[FUNC_MAX] = {};
+  hipStream_t m_Stream = nullptr;
+  void* m_cuParams = nullptr;  // device address of FParams
+
+  int mNumPoints = 0;
+  int mMaxPoints = 0;
+  int m_Frame = 0;
+  DataX m_Points;      // current, cell-sorted set
+  DataX m_PointsTemp;  // the other device set
+  DataX m_Accel;
+  FParams_t m_Params;
+
+  // device copy of both sets' pointer tables: [0] = set A, [1] = set B
+  void* m_Tables = nullptr;
+  int m_Parity = 0;  // which set m_Points holds
+  float* m_Clock = nullptr;
+
+  hipGraph_t m_Graph = nullptr;
+  hipGraphExec_t m_GraphExec = nullptr;
+  bool m_Capturing = false;
+  int m_GraphParity = 0;
+};

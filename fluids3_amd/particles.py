@@ -1,0 +1,234 @@
+"""Python mirror of the reference's Particles interface over the C ABI (libfluidsx.so).
+
+Method names follow fluids5.0/src/particles.h:35-70 (Restart, Run,
+InsertParticles, PrefixScanParticles, CountingSort, ComputePressure,
+ComputeForce, Advance, Retrieve ...).  Every call goes through the C ABI of
+include/fluidsx.h and runs the gfx950 code object; there is no CPU fallback:
+a missing library or a failing call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfluidsx.so")
+CODE_OBJECT = os.path.join(_HERE, "particles_gfx950.hsaco")
+
+_lib = None
+
+# every symbol include/fluidsx.h declares: name -> (restype, argtypes)
+_P = C.c_void_p
+SIGNATURES = {
+    "fx_scene_derive": (C.c_int, [C.POINTER(abi.Scene), C.POINTER(abi.FParams)]),
+    "fx_scene_spacing": (C.c_float, [C.POINTER(abi.Scene)]),
+    "fx_scene_fill": (C.c_int, [C.POINTER(abi.Scene), _P, _P, C.c_int32]),
+    "fx_create": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "fx_destroy": (None, [_P]),
+    "fx_restart": (C.c_int, [_P, C.POINTER(abi.Scene)]),
+    "fx_get_params": (C.c_int, [_P, C.POINTER(abi.FParams)]),
+    "fx_set_params": (C.c_int, [_P, C.POINTER(abi.FParams)]),
+    "fx_num_points": (C.c_int32, [_P]),
+    "fx_grid_total": (C.c_int32, [_P]),
+    "fx_buffer_stride": (C.c_int32, [C.c_int32]),
+    "fx_upload": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
+    "fx_download": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
+    "fx_download_accel": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
+    "fx_stage": (C.c_int, [_P, C.c_int32]),
+    "fx_run": (C.c_int, [_P, C.c_int32]),
+    "fx_synchronize": (C.c_int, [_P]),
+    "fx_scan": (C.c_int, [_P, _P, _P, C.c_int32]),
+    "fx_graph_capture": (C.c_int, [_P]),
+    "fx_graph_launch": (C.c_int, [_P, C.c_int32]),
+    "fx_profile_stages": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
+    "fx_time_steps": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    "fx_set_block_size": (C.c_int, [_P, C.c_int32]),
+    "fx_last_error": (C.c_char_p, []),
+    "fx_version": (C.c_char_p, []),
+}
+
+
+class FluidsError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded libfluidsx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FluidsError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, what):
+    if status != 0:
+        msg = lib().fx_last_error().decode(errors="replace")
+        raise FluidsError(f"{what} failed ({status}): {msg}")
+
+
+def derive_params(scene):
+    p = abi.FParams()
+    check(lib().fx_scene_derive(C.byref(scene), C.byref(p)), "fx_scene_derive")
+    return p
+
+
+def scene_spacing(scene):
+    return float(lib().fx_scene_spacing(C.byref(scene)))
+
+
+def fill_points(scene, n=None):
+    """Initial lattice of a scene on the host: (pos float32[n,3], clr uint32[n])."""
+    n = scene.num_points if n is None else n
+    pos = np.zeros((n, 3), np.float32)
+    clr = np.zeros(n, np.uint32)
+    got = lib().fx_scene_fill(C.byref(scene), pos.ctypes.data, clr.ctypes.data, n)
+    if got < 0:
+        check(got, "fx_scene_fill")
+    return pos[:got], clr[:got]
+
+
+class Particles:
+    """One simulation on the GPU: the reference's Particles (particles.h:29-96)."""
+
+    def __init__(self, code_object=CODE_OBJECT):
+        if not os.path.exists(code_object):
+            raise FluidsError(f"{code_object} is missing: build it with __graft_entry__.build()")
+        h = C.c_void_p()
+        check(lib().fx_create(code_object.encode(), C.byref(h)), "fx_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- set-up
+    def Restart(self, scene):
+        check(lib().fx_restart(self._h, C.byref(scene)), "fx_restart")
+
+    @property
+    def params(self):
+        p = abi.FParams()
+        check(lib().fx_get_params(self._h, C.byref(p)), "fx_get_params")
+        return p
+
+    def UpdateParams(self, p):
+        check(lib().fx_set_params(self._h, C.byref(p)), "fx_set_params")
+
+    def NumPoints(self):
+        return int(lib().fx_num_points(self._h))
+
+    def GridTotal(self):
+        return int(lib().fx_grid_total(self._h))
+
+    def SetBlockSize(self, threads):
+        check(lib().fx_set_block_size(self._h, threads), "fx_set_block_size")
+
+    # ---- stages (particles.cpp:382-394)
+    def InsertParticles(self):
+        check(lib().fx_stage(self._h, abi.STAGE_INSERT), "insert")
+
+    def PrefixScanParticles(self):
+        check(lib().fx_stage(self._h, abi.STAGE_PREFIX), "prefix")
+
+    def CountingSort(self):
+        check(lib().fx_stage(self._h, abi.STAGE_SORT), "sort")
+
+    def ComputePressure(self):
+        check(lib().fx_stage(self._h, abi.STAGE_PRESSURE), "pressure")
+
+    def ComputeForce(self):
+        check(lib().fx_stage(self._h, abi.STAGE_FORCE), "force")
+
+    def Advance(self):
+        """Advance and AdvanceTime (particles.cpp:413-418, 396-400)."""
+        check(lib().fx_stage(self._h, abi.STAGE_ADVANCE), "advance")
+
+    def Stage(self, k):
+        check(lib().fx_stage(self._h, k), f"stage {k}")
+
+    def Run(self, nsteps=1):
+        check(lib().fx_run(self._h, nsteps), "fx_run")
+
+    def Synchronize(self):
+        check(lib().fx_synchronize(self._h), "fx_synchronize")
+
+    def Scan(self, values):
+        """Exclusive prefix sum of a uint32 array with the grid-offset scan kernels."""
+        a = np.ascontiguousarray(values, dtype=np.uint32)
+        out = np.empty_like(a)
+        check(lib().fx_scan(self._h, a.ctypes.data, out.ctypes.data, a.shape[0]), "fx_scan")
+        return out
+
+    def CaptureGraph(self):
+        check(lib().fx_graph_capture(self._h), "fx_graph_capture")
+
+    def LaunchGraph(self, nlaunch):
+        check(lib().fx_graph_launch(self._h, nlaunch), "fx_graph_launch")
+
+    # ---- transfers
+    def Retrieve(self, buf, n=None):
+        """Device copy of particle buffer `buf` (FPOS..FID) as a numpy array."""
+        n = self.NumPoints() if n is None else n
+        dt, comp = abi.PBUF_LAYOUT[buf]
+        out = np.empty((n, comp) if comp > 1 else (n,), dtype=dt)
+        check(lib().fx_download(self._h, buf, out.ctypes.data, n), f"download {buf}")
+        return out
+
+    def Commit(self, buf, arr):
+        dt, comp = abi.PBUF_LAYOUT[buf]
+        a = np.ascontiguousarray(arr, dtype=dt)
+        n = a.shape[0]
+        if comp > 1 and a.shape[1:] != (comp,):
+            raise ValueError(f"buffer {buf} expects shape (n, {comp})")
+        check(lib().fx_upload(self._h, buf, a.ctypes.data, n), f"upload {buf}")
+
+    def RetrieveAccel(self, buf, n):
+        out = np.empty(n, dtype=np.uint32)
+        check(lib().fx_download_accel(self._h, buf, out.ctypes.data, n), f"download_accel {buf}")
+        return out
+
+    def state(self):
+        """All particle buffers and the grid counts/offsets, as numpy arrays."""
+        G = self.GridTotal()
+        st = {name: self.Retrieve(b) for name, b in (
+            ("pos", abi.FPOS), ("vel", abi.FVEL), ("veval", abi.FVEVAL), ("force", abi.FFORCE),
+            ("press", abi.FPRESS), ("clr", abi.FCLR), ("gcell", abi.FGCELL), ("gndx", abi.FGNDX),
+            ("id", abi.FID))}
+        st["cnt"] = self.RetrieveAccel(abi.AGRIDCNT, G + 1)
+        st["off"] = self.RetrieveAccel(abi.AGRIDOFF, G + 1)
+        st["grid"] = self.RetrieveAccel(abi.AGRID, self.NumPoints())
+        st["time"] = self.params.time
+        return st
+
+    # ---- timing
+    def ProfileStages(self, nsteps):
+        out = (C.c_double * (abi.NUM_STAGES + 1))()
+        check(lib().fx_profile_stages(self._h, nsteps, out), "fx_profile_stages")
+        return {**{abi.STAGE_NAMES[k]: out[k] for k in range(abi.NUM_STAGES)}, "step": out[abi.NUM_STAGES]}
+
+    def TimeSteps(self, nsteps, use_graph=True):
+        ms = C.c_double()
+        check(lib().fx_time_steps(self._h, nsteps, 1 if use_graph else 0, C.byref(ms)), "fx_time_steps")
+        return ms.value

@@ -1,0 +1,158 @@
+/*
+ * fluidsx.h -- C ABI of the MI355X SPH step (libfluidsx.so).
+ *
+ * Plain pointers and sizes only.  Every call returns 0 on success or a
+ * negative FX_E* code; fx_last_error() then holds a message.  The reference
+ * reports errors by printing and calling exit() (cuCheck,
+ * fluids5.0/helpers/common_defs.cpp:223-252; DataX exit(-9/-11/-12),
+ * fluids5.0/src/datax.cpp:76-79,114-117, dataptr.cpp:280-283); here they are
+ * status codes.
+ *
+ * Threading: like the reference (one host thread, one context), an
+ * fx_system is not thread-safe.  All device work of one fx_system is ordered
+ * on one HIP stream owned by it (default priority); the reference uses the
+ * NULL stream.
+ */
+#ifndef FLUIDSX_H
+#define FLUIDSX_H
+
+#include <stdint.h>
+
+#include "fluidsx_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FX_OK 0
+#define FX_EINVAL (-1)   /* bad argument */
+#define FX_EHIP (-2)     /* a HIP runtime call failed */
+#define FX_ESTATE (-3)   /* call not valid in the current state */
+#define FX_ERANGE (-4)   /* size beyond a documented limit */
+#define FX_EMISMATCH (-5) /* buffer layouts do not match (DataX exit -9/-11/-12) */
+
+/* per-function stages of one step, in Particles::Run order
+ * (fluids5.0/src/particles.cpp:382-394) */
+#define FX_STAGE_INSERT 0   /* Particles::InsertParticles, particles.cpp:716-730 */
+#define FX_STAGE_PREFIX 1   /* Particles::PrefixScanParticles, particles.cpp:774-815 */
+#define FX_STAGE_SORT 2     /* Particles::CountingSort, particles.cpp:832-842 */
+#define FX_STAGE_PRESSURE 3 /* Particles::ComputePressure, particles.cpp:852-857 */
+#define FX_STAGE_FORCE 4    /* Particles::ComputeForce, particles.cpp:914-919 */
+#define FX_STAGE_ADVANCE 5  /* Particles::Advance, particles.cpp:413-418 */
+#define FX_NUM_STAGES 6
+
+/*
+ * Scene description: what SetupDefaultParams / SetupExampleParams /
+ * AddPointsInVolume hard-code in the reference (particles.cpp:122-263,
+ * 318-379), as data.
+ */
+typedef struct fx_scene {
+  int32_t num_points;     /* N = mMaxPoints = mNumPoints (particles.cpp:95-119) */
+  int32_t reserved0;
+  uint64_t seed;          /* counter-based jitter RNG key */
+  float dt, sim_scale, smooth_radius, mass, rest_density, visc, int_stiff;
+  float bound_stiff, bound_damp, accel_limit, vel_limit;
+  float bound_slope, bound_friction, bound_wall_force, bound_wall_freq;
+  float pradius;      /* boundary contact radius (m) */
+  float grid_density; /* cell edge = 1.5 h / grid_density (particles.cpp:551) */
+  float grav_amt;
+  fx_float3 grav_dir;
+  fx_float3 bound_min, bound_max, init_min, init_max;
+  float spacing; /* lattice spacing (world); 0 = rest spacing, shrunk to fit N */
+  float jitter;  /* uniform jitter amplitude as a fraction of the spacing */
+} fx_scene;
+
+typedef struct fx_system fx_system;
+
+/* ---- host-only set-up (no GPU needed) ---- */
+
+/* Derives the full FParams_t of a scene: defaults, grid
+ * (Particles::RebuildAccelGrid, particles.cpp:548-626), kernel constants
+ * (Particles::UpdateParams, particles.cpp:657-679), launch geometry. */
+int fx_scene_derive(const fx_scene* scene, FParams_t* out);
+
+/* Lattice spacing the scene uses (rest spacing, or the largest spacing whose
+ * lattice in [init_min, init_max] holds num_points sites). */
+float fx_scene_spacing(const fx_scene* scene);
+
+/* Initial particle lattice (Particles::AddPointsInVolume,
+ * particles.cpp:318-379): y-layers bottom-up, x fastest then z, seeded
+ * uniform jitter, stop at n.  pos holds 3n floats, clr n uints.  Returns the
+ * number of particles written. */
+int fx_scene_fill(const fx_scene* scene, float* pos, uint32_t* clr, int32_t n);
+
+/* ---- lifecycle ---- */
+
+/* Particles::Initialize (particles.cpp:68-93): loads the code object by path,
+ * resolves the kernels and the FParams/FPnts/FPntTmp/FAccel symbols. */
+int fx_create(const char* code_object_path, fx_system** out);
+/* ~Particles (particles.cpp:44-51). */
+void fx_destroy(fx_system* sys);
+
+/* Particles::Restart (particles.cpp:95-119): derive params, allocate the
+ * particle and grid buffers, fill the lattice, upload everything. */
+int fx_restart(fx_system* sys, const fx_scene* scene);
+
+/* FParams_t currently in use (host copy). */
+int fx_get_params(const fx_system* sys, FParams_t* out);
+/* Particles::UpdateParams (particles.cpp:657-679): takes the input fields of
+ * p (rates, stiffness, walls, time ...), re-derives the kernel constants and
+ * uploads the 688-byte block.  The grid must not change (FX_ESTATE). */
+int fx_set_params(fx_system* sys, const FParams_t* p);
+
+int32_t fx_num_points(const fx_system* sys);
+int32_t fx_grid_total(const fx_system* sys);
+
+/* ---- buffers (DataX::Commit / Retrieve, datax.cpp:124-137) ---- */
+
+/* Element size of particle buffer id (FPOS..FID), 0 for an unknown id. */
+int32_t fx_buffer_stride(int32_t buf);
+/* Copy count elements of particle buffer buf between host and device.
+ * Synchronous with respect to the system's stream. */
+int fx_upload(fx_system* sys, int32_t buf, const void* host, int64_t count);
+int fx_download(fx_system* sys, int32_t buf, void* host, int64_t count);
+/* Grid buffers AGRID (N uints), AGRIDCNT / AGRIDOFF (gridTotal+1 uints). */
+int fx_download_accel(fx_system* sys, int32_t buf, void* host, int64_t count);
+
+/* ---- stepping ---- */
+
+/* One per-function stage (FX_STAGE_*), enqueued on the stream. */
+int fx_stage(fx_system* sys, int32_t stage);
+/* Particles::Run n times (stages in order, then AdvanceTime). */
+int fx_run(fx_system* sys, int32_t nsteps);
+int fx_synchronize(fx_system* sys);
+
+/* Exclusive prefix sum of n uints with the grid-offset scan kernels
+ * (prefixSum / prefixFixup, up to three levels), through temporary device
+ * buffers: the role of TestPrefixSum (fluids4.0/fluid_system.cpp:2120-2160).
+ * n <= 1024^3 - 1. */
+int fx_scan(fx_system* sys, const uint32_t* in, uint32_t* out, int32_t n);
+
+/* Captures two consecutive steps (the device sets alternate roles each step)
+ * into a HIP graph; fx_graph_launch replays it, advancing 2*nlaunch steps.
+ * The simulation time then lives on the device. */
+int fx_graph_capture(fx_system* sys);
+int fx_graph_launch(fx_system* sys, int32_t nlaunch);
+
+/* Runs nsteps eagerly with HIP events around every stage on the system's
+ * stream; ms_out[FX_NUM_STAGES] receives the mean milliseconds per stage and
+ * ms_out[FX_NUM_STAGES] the mean step time. */
+int fx_profile_stages(fx_system* sys, int32_t nsteps, double* ms_out);
+/* Wall time of nsteps: graph replays when use_graph, else eager fx_run;
+ * measured with HIP events on the system's stream.  */
+int fx_time_steps(fx_system* sys, int32_t nsteps, int32_t use_graph, double* ms_total);
+
+/* Launch geometry: threads per block of the per-particle kernels (64..1024,
+ * multiple of 64). */
+int fx_set_block_size(fx_system* sys, int32_t threads);
+
+/* Last error message of the calling thread. */
+const char* fx_last_error(void);
+const char* fx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUIDSX_H */

@@ -1,0 +1,142 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  The product path never loads it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+STAGE_INSERT, STAGE_SCAN, STAGE_SORT, STAGE_PRESSURE, STAGE_FORCE, STAGE_ADVANCE = 1, 2, 4, 8, 16, 32
+STAGE_ALL = 63
+# oracle stage bits in the order of the device's per-function stages
+STAGE_BITS = (STAGE_INSERT, STAGE_SCAN, STAGE_SORT, STAGE_PRESSURE, STAGE_FORCE, STAGE_ADVANCE)
+
+FIELDS = ("pos", "vel", "veval", "force", "press", "clr", "id", "gcell", "gndx", "cnt", "off", "grid")
+
+
+def build():
+    r = subprocess.run(["make", "-B", "-C", HERE], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("building the oracle failed:\n" + r.stdout + r.stderr)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        for name in ("orc_run_f32", "orc_run_f64"):
+            fn = getattr(L, name)
+            fn.restype = C.c_int
+            fn.argtypes = [P, C.c_int] + [P] * 13 + [C.c_int, C.c_int, C.c_int]
+        L.orc_time_f32.restype = C.c_double
+        L.orc_time_f32.argtypes = [P, C.c_int] + [P] * 13 + [C.c_int, C.c_int]
+        L.orc_derive.restype = None
+        L.orc_derive.argtypes = [P]
+        L.orc_pairs_grid.restype = C.c_int
+        L.orc_pairs_grid.argtypes = [P, C.c_int, P, P, P, P, P, P]
+        L.orc_pairs_brute.restype = C.c_int
+        L.orc_pairs_brute.argtypes = [P, C.c_int, P, P, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data
+
+
+def make_state(params, pos, vel=None, veval=None, clr=None, ids=None, dtype=np.float32, time=0.0):
+    """Initial state dict for n particles (velocities zero unless given)."""
+    n = pos.shape[0]
+    G = params.gridTotal
+    z3 = np.zeros((n, 3), dtype)
+    return {
+        "pos": np.array(pos, dtype), "vel": z3.copy() if vel is None else np.array(vel, dtype),
+        "veval": z3.copy() if veval is None else np.array(veval, dtype), "force": z3.copy(),
+        "press": np.zeros(n, dtype), "clr": np.zeros(n, np.uint32) if clr is None else np.array(clr, np.uint32),
+        "id": np.arange(n, dtype=np.uint32) if ids is None else np.array(ids, np.uint32),
+        "gcell": np.zeros(n, np.int32), "gndx": np.zeros(n, np.int32),
+        "cnt": np.zeros(G + 1, np.uint32), "off": np.zeros(G + 1, np.uint32), "grid": np.zeros(n, np.uint32),
+        "time": float(time),
+    }
+
+
+def convert(state, dtype):
+    out = {}
+    for k, v in state.items():
+        if k in ("pos", "vel", "veval", "force", "press"):
+            out[k] = np.array(v, dtype)
+        elif k == "time":
+            out[k] = float(v)
+        else:
+            out[k] = np.array(v, copy=True)
+    return out
+
+
+def run(params, state, nsteps=1, stages=STAGE_ALL, threads=1, fp64=False):
+    """Runs the oracle in place on a state dict (see make_state); returns the state."""
+    dt = np.float64 if fp64 else np.float32
+    s = state
+    for k in ("pos", "vel", "veval", "force"):
+        s[k] = np.ascontiguousarray(s[k], dtype=dt).reshape(-1, 3)
+    s["press"] = np.ascontiguousarray(s["press"], dtype=dt)
+    for k in ("clr", "id", "cnt", "off", "grid"):
+        s[k] = np.ascontiguousarray(s[k], dtype=np.uint32)
+    for k in ("gcell", "gndx"):
+        s[k] = np.ascontiguousarray(s[k], dtype=np.int32)
+    n = s["pos"].shape[0]
+    assert s["cnt"].shape[0] == params.gridTotal + 1
+    t = (C.c_double if fp64 else C.c_float)(s["time"])
+    fn = lib().orc_run_f64 if fp64 else lib().orc_run_f32
+    fn(C.byref(params), n, *[_ptr(s[k]) for k in FIELDS], C.byref(t), nsteps, stages, threads)
+    s["time"] = t.value
+    return s
+
+
+def time_steps(params, state, nsteps, threads=1):
+    """Seconds for nsteps FP32 oracle steps on `threads` host threads (state advanced in place)."""
+    s = run(params, state, 0)  # normalises dtypes
+    n = s["pos"].shape[0]
+    t = C.c_float(s["time"])
+    sec = lib().orc_time_f32(C.byref(params), n, *[_ptr(s[k]) for k in FIELDS], C.byref(t), nsteps, threads)
+    s["time"] = t.value
+    return sec
+
+
+def derive(params):
+    """Oracle restatement of RebuildAccelGrid + UpdateParams applied to a copy of params."""
+    from fluids3_amd.abi import params_copy
+
+    p = params_copy(params)
+    lib().orc_derive(C.byref(p))
+    return p
+
+
+def pairs_grid(params, state):
+    n = state["pos"].shape[0]
+    cand = np.zeros(n, np.int32)
+    pairs = np.zeros(n, np.int32)
+    pos = np.ascontiguousarray(state["pos"], np.float32)
+    lib().orc_pairs_grid(C.byref(params), n, _ptr(pos), _ptr(np.ascontiguousarray(state["gcell"], np.int32)),
+                         _ptr(np.ascontiguousarray(state["cnt"], np.uint32)),
+                         _ptr(np.ascontiguousarray(state["off"], np.uint32)), _ptr(cand), _ptr(pairs))
+    return cand, pairs
+
+
+def pairs_brute(params, pos):
+    pos = np.ascontiguousarray(pos, np.float32)
+    n = pos.shape[0]
+    pairs = np.zeros(n, np.int32)
+    dens = np.zeros(n, np.float64)
+    lib().orc_pairs_brute(C.byref(params), n, _ptr(pos), _ptr(pairs), _ptr(dens))
+    return pairs, dens
