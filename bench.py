@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Throughput of the SPH step on one MI355X: particle-steps/s at 4,000,000 particles.
+
+Workload (default): the 4,000,000-particle dam break in the box
+(-120,0,-60)..(120,120,60) with the initial block filling the left 40 % of the
+box (0.40x rest spacing, about 15x rest density), FP32, 20 warm-up steps then
+200 timed steps.  Inputs are resident in HBM before the timed region.
+
+Timed region: the K steps are captured as one HIP graph, with a HIP event
+before every stage and after the last, on the library's stream, and launched
+once; value = N * K / elapsed.  The per-stage event pairs of that same region
+give each kernel's mean duration, from which the roofline fraction of the
+dominant stage is computed with the algorithmic cost model:
+  bytes per particle-step: insert 20, sort 92, density 20, force 44,
+      advance 88 (264 in all), plus 16 per grid cell;
+  flops per particle-step: insert 6, density 8C+5P+2, force 8C+31P+2,
+      advance 60;
+C (stencil candidates) and P (in-radius pairs) per particle are measured on
+the same trajectory (the step is deterministic) every --stats-every steps.
+
+CPU baseline: the FP32 CPU oracle (a port of the reference's step) on the
+host's cores, on the state at the start of the timed window.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (matrix and packed vector)
+
+BYTES_PER_PARTICLE = {"insert": 20, "prefix": 0, "sort": 92, "pressure": 20, "force": 44, "advance": 88}
+BYTES_PER_CELL = {"insert": 4, "prefix": 8, "sort": 4, "pressure": 0, "force": 0, "advance": 0}
+
+
+def flops_per_particle(stage, C, P):
+    return {"insert": 6.0, "prefix": 0.0, "sort": 0.0, "pressure": 8 * C + 5 * P + 2,
+            "force": 8 * C + 31 * P + 2, "advance": 60.0}[stage]
+
+
+def describe(name, sc, N, G):
+    b0, b1 = sc.bound_min.tolist(), sc.bound_max.tolist()
+    i0, i1 = sc.init_min.tolist(), sc.init_max.tolist()
+    r = lambda v: "(" + ",".join(f"{x:g}" for x in v) + ")"
+    return (f"{name}: dam break, {N:,} particles, box {r(b0)}..{r(b1)}, block {r(i0)}..{r(i1)}, "
+            f"{G:,} grid cells")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg1_rest"])
+    ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly")
+    ap.add_argument("--stats-every", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every core this process may use")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from fluids3_amd import Particles, scenes
+
+    sc = scenes.PRESETS[args.workload]()
+    K, W = args.steps, args.warmup
+    ps = Particles()
+    ps.Restart(sc)
+    N, G = ps.NumPoints(), ps.GridTotal()
+    p = ps.params
+
+    # warm-up, then the timed region (one graph of K steps with per-stage events;
+    # eager launches with the same events if the graph cannot carry them)
+    use_graph = not args.no_graph
+    ps.Run(W)
+    ps.Synchronize()
+    try:
+        ms_total, stage_ms = ps.BenchSteps(K, use_graph=use_graph)
+    except Exception as e:
+        if not use_graph:
+            raise
+        print(f"graph-timed region unavailable ({e}); timing eager launches", file=sys.stderr)
+        use_graph = False
+        ps.Restart(sc)
+        ps.Run(W)
+        ps.Synchronize()
+        ms_total, stage_ms = ps.BenchSteps(K, use_graph=False)
+    value = N * K / (ms_total * 1e-3)
+    ms_step = ms_total / K
+
+    # same trajectory again (bit-reproducible), untimed, for C and P
+    ps.Restart(sc)
+    ps.Run(W)
+    start_state = ps.state() if not args.no_cpu_baseline else None
+    csum = psum = 0
+    samples = 0
+    for s in range(K):
+        if s % max(1, args.stats_every) == 0:
+            ps.InsertParticles()
+            ps.PrefixScanParticles()
+            ps.CountingSort()
+            c, q = ps.CountPairs()
+            csum += c
+            psum += q
+            samples += 1
+            ps.ComputePressure()
+            ps.ComputeForce()
+            ps.Advance()
+        else:
+            ps.Run(1)
+    ps.Synchronize()
+    C = csum / (samples * N)
+    P = psum / (samples * N)
+
+    stages = {}
+    for k, ms in stage_ms.items():
+        b = BYTES_PER_PARTICLE[k] * N + BYTES_PER_CELL[k] * (G + 1)
+        f = flops_per_particle(k, C, P) * N
+        stages[k] = {"ms": round(ms, 5), "GB_s": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None,
+                     "TFLOP_s": round(f / (ms * 1e-3) / 1e12, 3) if ms > 0 else None}
+    dom = max(stage_ms, key=stage_ms.get)
+    fl = flops_per_particle(dom, C, P) * N
+    by = BYTES_PER_PARTICLE[dom] * N + BYTES_PER_CELL[dom] * (G + 1)
+    t_dom = stage_ms[dom] * 1e-3
+    # the binding roof of the dominant stage under the cost model
+    if fl / (FP32_PEAK_TFLOPS * 1e12) >= by / (HBM_PEAK_GBS * 1e9):
+        ach = fl / t_dom / 1e12
+        roof = {"bound": "mfma", "compute_unit": "valu-fp32", "achieved": round(ach, 3),
+                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4)}
+    else:
+        ach = by / t_dom / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4)}
+    roof["kernel"] = {"insert": "insertParticles", "prefix": "prefixSum", "sort": "countingSortFull",
+                      "pressure": "computePressure", "force": "computeForce", "advance": "advanceParticles"}[dom]
+    roof["kernel_ms"] = round(stage_ms[dom], 5)
+    roof["traffic"] = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+
+        threads = args.cpu_threads or len(os.sched_getaffinity(0))
+        st = oracle.make_state(p, start_state["pos"], vel=start_state["vel"], veval=start_state["veval"],
+                               clr=start_state["clr"], ids=start_state["id"], time=start_state["time"])
+        sec = oracle.time_steps(p, st, args.cpu_steps, threads=threads)
+        cpu = {"value": round(N * args.cpu_steps / sec, 1), "unit": "particle-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{args.cpu_steps} FP32 oracle step(s) of the same {N:,}-particle state at the start "
+                         f"of the timed window on {threads} host threads ({sec:.2f} s)"}
+
+    out = {
+        "metric": "particle-steps/sec at 4M particles (1 GPU)" if N == 4_000_000 else "particle-steps/sec",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered lattice)",
+        "config": {"workload": describe(args.workload, sc, N, G), "particles": N, "grid_cells": G,
+                   "graph": use_graph, "candidates_per_particle": round(C, 2),
+                   "pairs_per_particle": round(P, 2), "stages": stages},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    ps.close()
+
+
+if __name__ == "__main__":
+    main()

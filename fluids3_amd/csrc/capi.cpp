@@ -141,6 +141,17 @@ int fx_time_steps(fx_system* s, int32_t nsteps, int32_t use_graph, double* ms_to
   return s->p.TimeSteps(nsteps, use_graph != 0, ms_total);
 }
 
+int fx_bench_steps(fx_system* s, int32_t nsteps, int32_t use_graph, double* ms_total, double* stage_ms) {
+  FX_CHECK_SYS(s);
+  return s->p.BenchSteps(nsteps, use_graph != 0, ms_total, stage_ms);
+}
+
+int fx_count_pairs(fx_system* s, int64_t* candidates, int64_t* pairs) {
+  FX_CHECK_SYS(s);
+  if (candidates == nullptr || pairs == nullptr) return fx::fail(FX_EINVAL, "fx_count_pairs: null argument");
+  return s->p.CountPairs(candidates, pairs);
+}
+
 int fx_set_block_size(fx_system* s, int32_t threads) {
   FX_CHECK_SYS(s);
   return s->p.SetBlockSize(threads);

@@ -16,7 +16,7 @@
 static const char* kKernelNames[FUNC_MAX] = {
     "insertParticles",   "countingSortFull", "computePressure", "computeForce",
     "advanceParticles",  "prefixSum",        "prefixFixup",     "countingSortIndex",
-    "fxFillUint",        "advanceParticlesClock", "fxClockTick"};
+    "fxFillUint",        "advanceParticlesClock", "fxClockTick",     "fxCountPairs"};
 
 Particles::Particles() { memset(&m_Params, 0, sizeof(m_Params)); }
 
@@ -27,6 +27,7 @@ Particles::~Particles() {
   m_Accel.DeleteAllBuffers();
   if (m_Tables) (void)hipFree(m_Tables);
   if (m_Clock) (void)hipFree(m_Clock);
+  if (m_Stats) (void)hipFree(m_Stats);
   if (m_Stream) (void)hipStreamDestroy(m_Stream);
   if (m_Module) (void)hipModuleUnload(m_Module);
 }
@@ -47,6 +48,7 @@ int Particles::Initialize(const char* path) {
   FX_HIP(hipStreamCreate(&m_Stream), "Initialize");
   FX_HIP(hipMalloc(&m_Tables, 2 * sizeof(cuDataX)), "Initialize");
   FX_HIP(hipMalloc((void**)&m_Clock, sizeof(float)), "Initialize");
+  FX_HIP(hipMalloc(&m_Stats, 2 * sizeof(unsigned long long)), "Initialize");
   m_Params.numThreads = 256;
   return FX_OK;
 }
@@ -460,6 +462,103 @@ int Particles::ProfileStages(int nsteps, double* ms_out) {
   if (st != FX_OK) return st;
   for (int k = 0; k <= FX_NUM_STAGES; k++) ms_out[k] /= nsteps;
   return FX_OK;
+}
+
+int Particles::CountPairs(int64_t* cand, int64_t* pairs) {
+  if (mNumPoints == 0 || m_Stats == nullptr) return fx::fail(FX_ESTATE, "count_pairs: call Restart first");
+  int four = 4;
+  unsigned zero = 0;
+  void* st = m_Stats;
+  void* fargs[] = {&st, &zero, &four};
+  FX_TRY(Launch(FUNC_FILL, 1, 64, fargs));
+  void* args[] = {&st, &mNumPoints};
+  FX_TRY(Launch(FUNC_COUNT_PAIRS, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args));
+  unsigned long long h[2] = {0, 0};
+  FX_HIP(hipStreamSynchronize(m_Stream), "count_pairs");
+  FX_HIP(hipMemcpyDtoH(h, m_Stats, sizeof(h)), "count_pairs");
+  *cand = (int64_t)h[0];
+  *pairs = (int64_t)h[1];
+  return FX_OK;
+}
+
+int Particles::BenchSteps(int nsteps, bool use_graph, double* ms_total, double* stage_ms) {
+  // nsteps steps with a HIP event before every stage and one after the last,
+  // all on m_Stream; with use_graph the whole region (events included) is one
+  // captured graph launched once.
+  if (mNumPoints == 0) return fx::fail(FX_ESTATE, "bench: call Restart first");
+  if (nsteps <= 0 || ms_total == nullptr || stage_ms == nullptr) return fx::fail(FX_EINVAL, "bench: bad arguments");
+  FX_HIP(hipStreamSynchronize(m_Stream), "bench");
+  const int nev = nsteps * FX_NUM_STAGES + 1;
+  std::vector<hipEvent_t> ev(nev, nullptr);
+  int st = FX_OK;
+  for (int k = 0; k < nev && st == FX_OK; k++)
+    if (hipEventCreate(&ev[k]) != hipSuccess) st = fx::fail(FX_EHIP, "bench: hipEventCreate");
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  const float time0 = m_Params.time;
+  const int parity0 = m_Parity;
+  if (st == FX_OK && use_graph) {
+    if (hipStreamBeginCapture(m_Stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      st = fx::fail(FX_EHIP, "bench: begin capture");
+    else {
+      m_Capturing = true;
+      for (int s = 0; s < nsteps && st == FX_OK; s++)
+        for (int k = 0; k < FX_NUM_STAGES && st == FX_OK; k++) {
+          // an external event node, so that the replay records a timestamp
+          if (hipEventRecordWithFlags(ev[s * FX_NUM_STAGES + k], m_Stream, hipEventRecordExternal) != hipSuccess)
+            st = fx::fail(FX_EHIP, "bench: event record in capture");
+          if (st == FX_OK) st = (k == FX_STAGE_ADVANCE) ? Advance() : Stage(k);
+        }
+      if (st == FX_OK && hipEventRecordWithFlags(ev[nev - 1], m_Stream, hipEventRecordExternal) != hipSuccess)
+        st = fx::fail(FX_EHIP, "bench: event record in capture");
+      m_Capturing = false;
+      hipError_t e = hipStreamEndCapture(m_Stream, &g);
+      if (st == FX_OK && e != hipSuccess) st = fx::hip_fail(e, "bench", "hipStreamEndCapture");
+      if (st == FX_OK && hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
+        st = fx::fail(FX_EHIP, "bench: instantiate");
+      if (st == FX_OK && hipMemcpyHtoD(m_Clock, &m_Params.time, sizeof(float)) != hipSuccess)
+        st = fx::fail(FX_EHIP, "bench: clock");
+      if (st == FX_OK && hipGraphLaunch(ge, m_Stream) != hipSuccess) st = fx::fail(FX_EHIP, "bench: launch");
+      if (st == FX_OK)
+        for (int s = 0; s < nsteps; s++) AdvanceTime();
+    }
+  } else if (st == FX_OK) {
+    for (int s = 0; s < nsteps && st == FX_OK; s++)
+      for (int k = 0; k < FX_NUM_STAGES && st == FX_OK; k++) {
+        if (hipEventRecord(ev[s * FX_NUM_STAGES + k], m_Stream) != hipSuccess)
+          st = fx::fail(FX_EHIP, "bench: event record");
+        if (st == FX_OK) st = Stage(k);
+      }
+    if (st == FX_OK && hipEventRecord(ev[nev - 1], m_Stream) != hipSuccess) st = fx::fail(FX_EHIP, "bench: event record");
+  }
+  if (st == FX_OK && hipStreamSynchronize(m_Stream) != hipSuccess) st = fx::fail(FX_EHIP, "bench: synchronize");
+  if (st == FX_OK) {
+    for (int k = 0; k < FX_NUM_STAGES; k++) stage_ms[k] = 0.0;
+    for (int s = 0; s < nsteps && st == FX_OK; s++)
+      for (int k = 0; k < FX_NUM_STAGES && st == FX_OK; k++) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ev[s * FX_NUM_STAGES + k], ev[s * FX_NUM_STAGES + k + 1]) != hipSuccess)
+          st = fx::fail(FX_EHIP, "bench: elapsed");
+        stage_ms[k] += ms;
+      }
+    float tot = 0.0f;
+    if (st == FX_OK && hipEventElapsedTime(&tot, ev[0], ev[nev - 1]) != hipSuccess) st = fx::fail(FX_EHIP, "bench: elapsed");
+    *ms_total = tot;
+    for (int k = 0; k < FX_NUM_STAGES; k++) stage_ms[k] /= nsteps;
+  }
+  if (st != FX_OK && use_graph) {
+    // nothing of a failed capture ran: restore the host view of the sets
+    m_Params.time = time0;
+    if (m_Parity != parity0) {
+      (void)m_Points.SwapGPU(&m_PointsTemp);
+      m_Parity = parity0;
+    }
+  }
+  if (ge) (void)hipGraphExecDestroy(ge);
+  if (g) (void)hipGraphDestroy(g);
+  for (auto e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return st;
 }
 
 int Particles::TimeSteps(int nsteps, bool use_graph, double* ms_total) {

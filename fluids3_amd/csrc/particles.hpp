@@ -29,7 +29,8 @@
 #define FUNC_FILL 8
 #define FUNC_ADVANCE_CLOCK 9
 #define FUNC_CLOCK_TICK 10
-#define FUNC_MAX 11
+#define FUNC_COUNT_PAIRS 11
+#define FUNC_MAX 12
 
 class Particles {
  public:
@@ -74,6 +75,8 @@ class Particles {
   // timing
   int ProfileStages(int nsteps, double* ms_out);
   int TimeSteps(int nsteps, bool use_graph, double* ms_total);
+  int BenchSteps(int nsteps, bool use_graph, double* ms_total, double* stage_ms);
+  int CountPairs(int64_t* cand, int64_t* pairs);
 
   int NumPoints() const { return mNumPoints; }
   const FParams_t& Params() const { return m_Params; }
@@ -102,6 +105,7 @@ class Particles {
   void* m_Tables = nullptr;
   int m_Parity = 0;  // which set m_Points holds
   float* m_Clock = nullptr;
+  void* m_Stats = nullptr;  // two uint64 counters of fxCountPairs
 
   hipGraph_t m_Graph = nullptr;
   hipGraphExec_t m_GraphExec = nullptr;

@@ -297,6 +297,49 @@ extern "C" __global__ void computeForce(int pnum) {
   pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{fx, fy, fz};
 }
 
+// ------------------------------------------------------ neighbour statistics
+
+// Sums over all in-range particles of the stencil candidates (members of the
+// stencil cells, self included) and of the in-radius pairs (0 < d2 < rd2),
+// on the cell-sorted set: out[0] += C, out[1] += P.  The measurement behind
+// the flop model of the density and force stages; the role of the
+// reference's neighbour-count validation (fluids4.0/fluid_system.cpp:959-1011).
+extern "C" __global__ void fxCountPairs(unsigned long long* out, int pnum) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long nc = 0, np = 0;
+  if (i < pnum) {
+    const int gc = pbuf<const int>(FPnts, FGCELL)[i];
+    if (gc != FX_GRID_UNDEF) {
+      const fx_float3* __restrict__ pos = pbuf<const fx_float3>(FPnts, FPOS);
+      const uint* __restrict__ off = pbuf<const uint>(FAccel, AGRIDOFF);
+      const uint* __restrict__ cnt = pbuf<const uint>(FAccel, AGRIDCNT);
+      const fx_float3 p = pos[i];
+      const int base = gc - (FParams.gridRes.z + 1) * FParams.gridRes.x - 1;
+      const int srch = FParams.gridSrch;
+      for (int r = 0; r < srch * srch; r++) {
+        int jb, je;
+        row_range(base, r, srch, off, cnt, jb, je);
+        nc += (unsigned long long)(je - jb);
+        for (int j = jb; j < je; j++) {
+          const fx_float3 q = pos[j];
+          const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+          const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+          np += (dsq < FParams.rd2 && dsq > 0.0f) ? 1u : 0u;
+        }
+      }
+    }
+  }
+  // wave64 reduction, one atomic per wave
+  for (int o = 32; o > 0; o >>= 1) {
+    nc += __shfl_xor(nc, o, 64);
+    np += __shfl_xor(np, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && (nc | np) != 0) {
+    atomicAdd(&out[0], nc);
+    atomicAdd(&out[1], np);
+  }
+}
+
 // ----------------------------------------------------------------- advance
 
 // Integrator (particles.cu:237-322): penalty walls on bound_min/bound_max

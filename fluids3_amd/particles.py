@@ -44,6 +44,8 @@ SIGNATURES = {
     "fx_graph_launch": (C.c_int, [_P, C.c_int32]),
     "fx_profile_stages": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),
     "fx_time_steps": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    "fx_bench_steps": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "fx_count_pairs": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "fx_set_block_size": (C.c_int, [_P, C.c_int32]),
     "fx_last_error": (C.c_char_p, []),
     "fx_version": (C.c_char_p, []),
@@ -227,6 +229,19 @@ class Particles:
         out = (C.c_double * (abi.NUM_STAGES + 1))()
         check(lib().fx_profile_stages(self._h, nsteps, out), "fx_profile_stages")
         return {**{abi.STAGE_NAMES[k]: out[k] for k in range(abi.NUM_STAGES)}, "step": out[abi.NUM_STAGES]}
+
+    def BenchSteps(self, nsteps, use_graph=True):
+        """(total ms, {stage: mean ms}) of nsteps steps timed with HIP events on the library stream."""
+        tot = C.c_double()
+        st = (C.c_double * abi.NUM_STAGES)()
+        check(lib().fx_bench_steps(self._h, nsteps, 1 if use_graph else 0, C.byref(tot), st), "fx_bench_steps")
+        return tot.value, {abi.STAGE_NAMES[k]: st[k] for k in range(abi.NUM_STAGES)}
+
+    def CountPairs(self):
+        """(candidates, pairs) summed over particles on the current cell-sorted state."""
+        c, p = C.c_int64(), C.c_int64()
+        check(lib().fx_count_pairs(self._h, C.byref(c), C.byref(p)), "fx_count_pairs")
+        return c.value, p.value
 
     def TimeSteps(self, nsteps, use_graph=True):
         ms = C.c_double()

@@ -143,6 +143,18 @@ int fx_profile_stages(fx_system* sys, int32_t nsteps, double* ms_out);
  * measured with HIP events on the system's stream.  */
 int fx_time_steps(fx_system* sys, int32_t nsteps, int32_t use_graph, double* ms_total);
 
+/* Benchmark region: nsteps steps with a HIP event before every stage and
+ * after the last, all on the system's stream; with use_graph the whole region
+ * (events included) is captured as one graph and launched once.  ms_total
+ * receives the region's time, stage_ms[FX_NUM_STAGES] the mean per stage. */
+int fx_bench_steps(fx_system* sys, int32_t nsteps, int32_t use_graph, double* ms_total, double* stage_ms);
+
+/* Neighbour statistics of the current cell-sorted state (call between the
+ * sort and the next advance): the sum over in-range particles of stencil
+ * candidates and of in-radius pairs (0 < d^2 < rd2); the role of the
+ * reference's neighbour-count validation (fluids4.0/fluid_system.cpp:959-1011). */
+int fx_count_pairs(fx_system* sys, int64_t* candidates, int64_t* pairs);
+
 /* Launch geometry: threads per block of the per-particle kernels (64..1024,
  * multiple of 64). */
 int fx_set_block_size(fx_system* sys, int32_t threads);
