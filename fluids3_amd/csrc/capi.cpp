@@ -152,6 +152,16 @@ int fx_count_pairs(fx_system* s, int64_t* candidates, int64_t* pairs) {
   return s->p.CountPairs(candidates, pairs);
 }
 
+int fx_set_stage_block_size(fx_system* s, int32_t stage, int32_t threads) {
+  FX_CHECK_SYS(s);
+  return s->p.SetStageBlockSize(stage, threads);
+}
+
+int32_t fx_stage_block_size(const fx_system* s, int32_t stage) {
+  if (s == nullptr || stage < 0 || stage >= FX_NUM_STAGES) return 0;
+  return s->p.StageBlockSize(stage);
+}
+
 int fx_set_block_size(fx_system* s, int32_t threads) {
   FX_CHECK_SYS(s);
   return s->p.SetBlockSize(threads);

@@ -60,6 +60,13 @@ int Particles::Launch(int fid, unsigned blocks, unsigned threads, void** args) {
   return FX_OK;
 }
 
+int Particles::LaunchP(int fid, int stage, void** args) {
+  const int waves = (mNumPoints + 63) / 64 * 64;
+  const int threads = m_StageThreads[stage] < waves ? m_StageThreads[stage] : waves;
+  if (threads <= 0) return FX_OK;
+  return Launch(fid, (unsigned)((mNumPoints + threads - 1) / threads), (unsigned)threads, args);
+}
+
 int Particles::Restart(const fx_scene& sc) {
   if (m_Module == nullptr) return fx::fail(FX_ESTATE, "Restart: call Initialize first");
   FX_HIP(hipStreamSynchronize(m_Stream), "Restart");
@@ -142,6 +149,8 @@ int Particles::RebuildAccelGrid() {
   FX_TRY(m_Accel.AddBuffer(AAUXSCAN1, "scan1", sizeof(unsigned), numElem2, DT_CUMEM));
   FX_TRY(m_Accel.AddBuffer(AAUXARRAY2, "aux2", sizeof(unsigned), numElem3, DT_CUMEM));
   FX_TRY(m_Accel.AddBuffer(AAUXSCAN2, "scan2", sizeof(unsigned), numElem3, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(ANBRBITS, "nbrbits", sizeof(unsigned), (uint64_t)FX_NBR_WORDS * mMaxPoints, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(AFREC, "frec", 32, mMaxPoints, DT_CUMEM));
   for (int b = 0; b <= AAUXSCAN2; b++) m_Accel.SetBufferUsage(b, DT_UINT);
   FX_TRY(UpdateGPUAccess());
   return FX_OK;
@@ -187,7 +196,17 @@ int Particles::SetBlockSize(int threads) {
     return fx::fail(FX_EINVAL, "SetBlockSize: %d is not a multiple of 64 in [64, 1024]", threads);
   ReleaseGraph();
   m_Params.numThreads = threads;
+  for (int k = 0; k < FX_NUM_STAGES; k++) m_StageThreads[k] = threads;
   if (mMaxPoints > 0) fx::set_geometry(m_Params, threads);
+  return FX_OK;
+}
+
+int Particles::SetStageBlockSize(int stage, int threads) {
+  if (stage < 0 || stage >= FX_NUM_STAGES) return fx::fail(FX_EINVAL, "SetStageBlockSize: bad stage %d", stage);
+  if (threads < 64 || threads > 1024 || threads % 64 != 0)
+    return fx::fail(FX_EINVAL, "SetStageBlockSize: %d is not a multiple of 64 in [64, 1024]", threads);
+  ReleaseGraph();
+  m_StageThreads[stage] = threads;
   return FX_OK;
 }
 
@@ -207,7 +226,7 @@ int Particles::InsertParticles() {
   void* fargs[] = {&cnt, &zero, &ncell};
   FX_TRY(Launch(FUNC_FILL, (unsigned)((ncell + 255) / 256), 256, fargs));
   void* args[] = {&mNumPoints};
-  return Launch(FUNC_INSERT, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args);
+  return LaunchP(FUNC_INSERT, FX_STAGE_INSERT, args);
 }
 
 int Particles::PrefixScanParticles() {
@@ -257,18 +276,18 @@ int Particles::CountingSort() {
   m_Parity ^= 1;
   FX_TRY(PublishTables());
   void* args[] = {&mNumPoints};
-  FX_TRY(Launch(FUNC_SORT_INDEX, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args));
-  return Launch(FUNC_COUNTING_SORT, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args);
+  FX_TRY(LaunchP(FUNC_SORT_INDEX, FX_STAGE_SORT, args));
+  return LaunchP(FUNC_COUNTING_SORT, FX_STAGE_SORT, args);
 }
 
 int Particles::ComputePressure() {
   void* args[] = {&mNumPoints};
-  return Launch(FUNC_COMPUTE_PRESS, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args);
+  return LaunchP(FUNC_COMPUTE_PRESS, FX_STAGE_PRESSURE, args);
 }
 
 int Particles::ComputeForce() {
   void* args[] = {&mNumPoints};
-  return Launch(FUNC_COMPUTE_FORCE, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args);
+  return LaunchP(FUNC_COMPUTE_FORCE, FX_STAGE_FORCE, args);
 }
 
 int Particles::Advance() {
@@ -277,13 +296,13 @@ int Particles::Advance() {
   int n = m_Params.pnum;
   if (m_Capturing) {
     void* args[] = {&m_Clock, &dt, &ss, &n};
-    FX_TRY(Launch(FUNC_ADVANCE_CLOCK, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args));
+    FX_TRY(LaunchP(FUNC_ADVANCE_CLOCK, FX_STAGE_ADVANCE, args));
     void* targs[] = {&m_Clock, &dt};
     return Launch(FUNC_CLOCK_TICK, 1, 64, targs);
   }
   float time = m_Params.time;
   void* args[] = {&time, &dt, &ss, &n};
-  return Launch(FUNC_ADVANCE, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args);
+  return LaunchP(FUNC_ADVANCE, FX_STAGE_ADVANCE, args);
 }
 
 void Particles::AdvanceTime() {
@@ -472,7 +491,7 @@ int Particles::CountPairs(int64_t* cand, int64_t* pairs) {
   void* fargs[] = {&st, &zero, &four};
   FX_TRY(Launch(FUNC_FILL, 1, 64, fargs));
   void* args[] = {&st, &mNumPoints};
-  FX_TRY(Launch(FUNC_COUNT_PAIRS, (unsigned)m_Params.numBlocks, (unsigned)m_Params.numThreads, args));
+  FX_TRY(LaunchP(FUNC_COUNT_PAIRS, FX_STAGE_PRESSURE, args));
   unsigned long long h[2] = {0, 0};
   FX_HIP(hipStreamSynchronize(m_Stream), "count_pairs");
   FX_HIP(hipMemcpyDtoH(h, m_Stats, sizeof(h)), "count_pairs");

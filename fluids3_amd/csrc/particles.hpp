@@ -48,6 +48,8 @@ class Particles {
   int UpdateGPUAccess();
   int SetParams(const FParams_t& p);
   int SetBlockSize(int threads);
+  int SetStageBlockSize(int stage, int threads);
+  int StageBlockSize(int stage) const { return m_StageThreads[stage]; }
 
   // the step (particles.cpp:382-400)
   int Run();
@@ -84,6 +86,8 @@ class Particles {
 
  private:
   int Launch(int fid, unsigned blocks, unsigned threads, void** args);
+  // one thread per particle, with the block size chosen for the stage
+  int LaunchP(int fid, int stage, void** args);
   int ScanLevels(void* array1, void* scan1, void* array2, void* scan2, void* array3, void* scan3, int n);
   int PublishTables();
   void ReleaseGraph();
@@ -111,4 +115,7 @@ class Particles {
   hipGraphExec_t m_GraphExec = nullptr;
   bool m_Capturing = false;
   int m_GraphParity = 0;
+  // threads per block of each stage's per-particle kernels; the force stage
+  // runs one independent wave per block
+  int m_StageThreads[FX_NUM_STAGES] = {256, 256, 256, 256, 64, 256};
 };

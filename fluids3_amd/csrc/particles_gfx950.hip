@@ -12,6 +12,8 @@
 // FP32 results are then reproducible run to run and comparable bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "fluidsx_params.h"
 
 typedef unsigned int uint;
@@ -26,9 +28,14 @@ __constant__ cuDataX FAccel;
 
 // ---------------------------------------------------------------- helpers
 
+// Buffer id -> pointer.  The table holds global-memory addresses; passing
+// them through the global address space lets the compiler emit global_*
+// instructions instead of flat ones (flat accesses also count against the
+// LDS wait counter and force coarser waits).
 template <class T>
 __device__ __forceinline__ T* pbuf(const cuDataX& d, int id) {
-  return reinterpret_cast<T*>(d.mbuf[id]);
+  typedef __attribute__((address_space(1))) T gT;
+  return (T*)(gT*)(d.mbuf[id]);
 }
 
 __device__ __forceinline__ fx_float3 ld3(const fx_float3* __restrict__ p, int i) {
@@ -73,28 +80,49 @@ extern "C" __global__ void fxFillUint(uint* __restrict__ p, uint v, int n) {
 // g >= 1; trunc(g) <= M iff g < M+1), which is the same predicate for every
 // finite input and sends NaN to GRID_UNDEF.  Out-of-range particles are
 // counted in slot gridTotal (the tail bucket).
+//
+// The counting is aggregated per wave: the particles arrive sorted by cell
+// from the previous step, so the 64 lanes of a wave fall into a few cells;
+// lanes of one cell take a single atomicAdd of their number and their slot
+// is the returned base plus their rank among those lanes.
+__device__ __forceinline__ uint lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint)mask, 0u));
+}
+
 extern "C" __global__ void insertParticles(int pnum) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= pnum) return;
-  const fx_float3 p = ld3(pbuf<const fx_float3>(FPnts, FPOS), i);
-  const float gx = (p.x - FParams.gridMin.x) * FParams.gridDelta.x;
-  const float gy = (p.y - FParams.gridMin.y) * FParams.gridDelta.y;
-  const float gz = (p.z - FParams.gridMin.z) * FParams.gridDelta.z;
-  int* gcell = pbuf<int>(FPnts, FGCELL);
-  int* gndx = pbuf<int>(FPnts, FGNDX);
-  uint* cnt = pbuf<uint>(FAccel, AGRIDCNT);
-  const bool in = gx >= 1.0f && gx < (float)(FParams.gridScanMax.x + 1) &&
-                  gy >= 1.0f && gy < (float)(FParams.gridScanMax.y + 1) &&
-                  gz >= 1.0f && gz < (float)(FParams.gridScanMax.z + 1);
-  int gs;
-  if (in) {
-    gs = ((int)gy * FParams.gridRes.z + (int)gz) * FParams.gridRes.x + (int)gx;
-    gcell[i] = gs;
-  } else {
-    gs = FParams.gridTotal;
-    gcell[i] = FX_GRID_UNDEF;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < pnum;
+  int gs = 0;
+  if (active) {
+    const fx_float3 p = ld3(pbuf<const fx_float3>(FPnts, FPOS), i);
+    const float gx = (p.x - FParams.gridMin.x) * FParams.gridDelta.x;
+    const float gy = (p.y - FParams.gridMin.y) * FParams.gridDelta.y;
+    const float gz = (p.z - FParams.gridMin.z) * FParams.gridDelta.z;
+    const bool in = gx >= 1.0f && gx < (float)(FParams.gridScanMax.x + 1) &&
+                    gy >= 1.0f && gy < (float)(FParams.gridScanMax.y + 1) &&
+                    gz >= 1.0f && gz < (float)(FParams.gridScanMax.z + 1);
+    if (in) {
+      gs = ((int)gy * FParams.gridRes.z + (int)gz) * FParams.gridRes.x + (int)gx;
+      pbuf<int>(FPnts, FGCELL)[i] = gs;
+    } else {
+      gs = FParams.gridTotal;
+      pbuf<int>(FPnts, FGCELL)[i] = FX_GRID_UNDEF;
+    }
   }
-  gndx[i] = (int)atomicAdd(&cnt[gs], 1u);
+  uint* cnt = pbuf<uint>(FAccel, AGRIDCNT);
+  unsigned long long todo = __ballot(active);
+  uint slot = 0;
+  while (todo != 0ull) {  // one pass per distinct slot in the wave
+    const int leader = __ffsll((long long)todo) - 1;
+    const int v = __builtin_amdgcn_readlane(gs, leader);
+    const unsigned long long same = __ballot(active && gs == v) & todo;
+    uint base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(&cnt[v], (uint)__popcll(same));
+    base = (uint)__builtin_amdgcn_readlane((int)base, leader);
+    if (active && gs == v) slot = base + lanes_below(same);
+    todo &= ~same;
+  }
+  if (active) pbuf<int>(FPnts, FGNDX)[i] = (int)slot;
 }
 
 // ------------------------------------------------------------- prefix scan
@@ -209,38 +237,152 @@ __device__ __forceinline__ void row_range(int base, int r, int srch, const uint*
   je = (int)(off[c1] + cnt[c1]);
 }
 
-// Density (particles.cu:120-157): rho_i = m * poly6 * sum_{0<d2<rd2}
-// ((rd2 - d2) * d2s)^3 with d2 in world units, stored in FPRESS; an isolated
-// particle gets 1.0.
-extern "C" __global__ void computePressure(int pnum) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= pnum) return;
-  const int gc = pbuf<const int>(FPnts, FGCELL)[i];
-  if (gc == FX_GRID_UNDEF) return;
-  const fx_float3* __restrict__ pos = pbuf<const fx_float3>(FPnts, FPOS);
-  const uint* __restrict__ off = pbuf<const uint>(FAccel, AGRIDOFF);
-  const uint* __restrict__ cnt = pbuf<const uint>(FAccel, AGRIDCNT);
-  const fx_float3 p = pos[i];
+// Wave-wide max / min with DPP row shifts and row broadcasts; lane 63 ends
+// up with the reduction of all 64 lanes and is read back as a scalar.
+template <bool kMax>
+__device__ __forceinline__ int wave_reduce(int v) {
+  const int id = kMax ? INT_MIN : INT_MAX;
+  int t;
+#define FX_RED_STEP(ctrl, rmask)                                         \
+  t = __builtin_amdgcn_update_dpp(id, v, ctrl, rmask, 0xf, false);      \
+  v = kMax ? max(v, t) : min(v, t);
+  FX_RED_STEP(0x111, 0xf)  // row_shr:1
+  FX_RED_STEP(0x112, 0xf)  // row_shr:2
+  FX_RED_STEP(0x114, 0xf)  // row_shr:4
+  FX_RED_STEP(0x118, 0xf)  // row_shr:8
+  FX_RED_STEP(0x142, 0xa)  // row_bcast:15
+  FX_RED_STEP(0x143, 0xc)  // row_bcast:31
+#undef FX_RED_STEP
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_max(int v) { return wave_reduce<true>(v); }
+__device__ __forceinline__ int wave_min(int v) { return wave_reduce<false>(v); }
+
+// Tiling.  A block owns the consecutive sorted particles [h0, h1) (in-range
+// particles only; out-of-range ones sit in the tail).  Their cells form one
+// contiguous index range [cf, cl], so for every stencil row the candidates of
+// all the block's particles form one contiguous run of sorted particles,
+// from the row start of cf to the row end of cl.  The block stages that run
+// in LDS in chunks of TILE_P with coalesced loads; each lane then reads the
+// part of the chunk that falls in its own row window.  Chunks are visited in
+// increasing order, so every lane still sees its candidates in the
+// reference's order.
+#define TILE_P 1024  // float4 (pos, 0) per candidate, 16 KB
+
+struct BlockSpan {
+  int h1;      // end of the block's in-range particles
+  int cf, cl;  // first and last cell of the block
+};
+
+__device__ __forceinline__ bool block_span(int h0, const int* __restrict__ gcell,
+                                           const uint* __restrict__ off, BlockSpan& s) {
+  const int nvalid = (int)off[FParams.gridTotal];  // start of the out-of-range tail
+  if (h0 >= nvalid) return false;
+  s.h1 = min(h0 + (int)blockDim.x, nvalid);
+  s.cf = gcell[h0];
+  s.cl = gcell[s.h1 - 1];
+  return true;
+}
+
+// Number of bitmask words a particle of cell gc uses: one word-aligned run of
+// ceil(len/32) words per stencil row.
+__device__ __forceinline__ int nbr_words(int gc, const uint* __restrict__ off,
+                                         const uint* __restrict__ cnt) {
   const int base = gc - (FParams.gridRes.z + 1) * FParams.gridRes.x - 1;
   const int srch = FParams.gridSrch;
-  const float rd2 = FParams.rd2, d2 = FParams.d2;
-  float sum = 0.0f;
+  int w = 0;
   for (int r = 0; r < srch * srch; r++) {
     int jb, je;
     row_range(base, r, srch, off, cnt, jb, je);
-    for (int j = jb; j < je; j++) {
-      const fx_float3 q = pos[j];
-      const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-      const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      if (dsq < rd2 && dsq > 0.0f) {
-        const float t = (rd2 - dsq) * d2;
-        sum = fmaf(t * t, t, sum);
+    w += (je - jb + 31) >> 5;
+  }
+  return w;
+}
+
+// Density (particles.cu:120-157): rho_i = m * poly6 * sum_{0<d2<rd2}
+// ((rd2 - d2) * d2s)^3 with d2 in world units, stored in FPRESS; an isolated
+// particle gets 1.0.
+//
+// When the host provides the ANBRBITS and AFREC buffers the kernel also
+// records, for the force stage, one bit per stencil candidate (set for the
+// in-radius pairs; row r of particle i occupies words W_r .. W_r+ceil(len/32)-1
+// at [w * N + i]) and the particle's (pos, rho), (veval, 0) record.  A
+// particle whose candidates need more than FX_NBR_WORDS words records no bits.
+extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
+  __shared__ float4 tile[TILE_P];
+  const int t = threadIdx.x;
+  const int h0 = blockIdx.x * blockDim.x;
+  const int i = h0 + t;
+  const int* __restrict__ gcell = pbuf<const int>(FPnts, FGCELL);
+  const fx_float3* __restrict__ pos = pbuf<const fx_float3>(FPnts, FPOS);
+  const uint* __restrict__ off = pbuf<const uint>(FAccel, AGRIDOFF);
+  const uint* __restrict__ cnt = pbuf<const uint>(FAccel, AGRIDCNT);
+  uint* __restrict__ bits = pbuf<uint>(FAccel, ANBRBITS);
+  const bool wbits = bits != nullptr && FAccel.mbuf[AFREC] != 0;
+  BlockSpan S;
+  if (h0 >= pnum || !block_span(h0, gcell, off, S)) return;  // uniform per block
+  const bool act = i < S.h1;
+  const int gc = act ? gcell[i] : S.cf;
+  fx_float3 p = fx_float3{0.0f, 0.0f, 0.0f};
+  if (act) p = pos[i];
+  const int shift = (FParams.gridRes.z + 1) * FParams.gridRes.x + 1;
+  const int srch = FParams.gridSrch;
+  const float rd2 = FParams.rd2, d2 = FParams.d2;
+  const size_t N = (size_t)pnum;
+  const bool rec = wbits && act && nbr_words(gc, off, cnt) <= FX_NBR_WORDS;
+  float sum = 0.0f;
+  int W = 0;  // first word of the current row
+  for (int r = 0; r < srch * srch; r++) {
+    const int adj = FParams.gridAdj[r * srch] - shift;
+    const int lo = (int)off[S.cf + adj];
+    const int hi = (int)(off[S.cl + adj + srch - 1] + cnt[S.cl + adj + srch - 1]);
+    int jb = 0, je = 0;
+    if (act) {
+      jb = (int)off[gc + adj];
+      je = (int)(off[gc + adj + srch - 1] + cnt[gc + adj + srch - 1]);
+    }
+    uint acc = 0;
+    for (int c0 = lo; c0 < hi; c0 += TILE_P) {
+      const int n = min(TILE_P, hi - c0);
+      for (int k = t; k < n; k += blockDim.x) {
+        const fx_float3 q = pos[c0 + k];
+        tile[k] = make_float4(q.x, q.y, q.z, 0.0f);
       }
+      __syncthreads();
+      const int a = max(jb, c0) - c0, b = min(je, c0 + n) - c0;
+      for (int k = a; k < b; k++) {
+        const float4 q = tile[k];
+        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        const bool pair = dsq < rd2 && dsq > 0.0f;
+        if (pair) {
+          const float tt = (rd2 - dsq) * d2;
+          sum = fmaf(tt * tt, tt, sum);
+        }
+        if (rec) {
+          const int bp = c0 + k - jb;  // bit position in the row window
+          acc |= (pair ? 1u : 0u) << (bp & 31);
+          if ((bp & 31) == 31 || c0 + k == je - 1) {
+            bits[(size_t)(W + (bp >> 5)) * N + i] = acc;
+            acc = 0;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    W += (je - jb + 31) >> 5;
+  }
+  if (act) {
+    sum = sum * FParams.pmass * FParams.poly6kern;
+    if (sum == 0.0f) sum = 1.0f;
+    pbuf<float>(FPnts, FPRESS)[i] = sum;
+    if (wbits) {
+      const fx_float3 v = pbuf<const fx_float3>(FPnts, FVEVAL)[i];
+      float4* fr = pbuf<float4>(FAccel, AFREC);
+      fr[2 * (size_t)i] = make_float4(p.x, p.y, p.z, sum);
+      fr[2 * (size_t)i + 1] = make_float4(v.x, v.y, v.z, 0.0f);
     }
   }
-  sum = sum * FParams.pmass * FParams.poly6kern;
-  if (sum == 0.0f) sum = 1.0f;
-  pbuf<float>(FPnts, FPRESS)[i] = sum;
 }
 
 // ------------------------------------------------------------------- force
@@ -249,52 +391,189 @@ extern "C" __global__ void computePressure(int pnum) {
 // c = h - r, p = (rho - rho0) * k,
 // pterm = (ss * -0.5) * c * spiky * (pi + pj) / r,
 // F += (pterm * dist + vterm * (veval_j - veval_i)) * c * (1 / (rho_i rho_j)).
-extern "C" __global__ void computeForce(int pnum) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= pnum) return;
-  const int gc = pbuf<const int>(FPnts, FGCELL)[i];
-  if (gc == FX_GRID_UNDEF) return;
-  const fx_float3* __restrict__ pos = pbuf<const fx_float3>(FPnts, FPOS);
-  const fx_float3* __restrict__ vev = pbuf<const fx_float3>(FPnts, FVEVAL);
-  const float* __restrict__ rho = pbuf<const float>(FPnts, FPRESS);
+//
+// With the density stage's bitmask the kernel never re-tests candidates.
+// Each lane turns the set bits of its words into neighbour indices in a
+// per-lane ring in LDS; the wave then evaluates queued pairs in batches: a
+// batch takes from every lane as many pairs as the shortest active queue
+// holds (more only when a queue must make room), so the expensive pair term
+// (square root, two IEEE divisions) runs with nearly every lane busy.  Each
+// lane adds its own pairs in candidate order.  Waves are independent: no
+// block-wide barrier.
+#define FQ_SLOTS 1024  // ring slots per block (16 per lane at 64 threads)
+
+struct ForceI {
+  float px, py, pz, vx, vy, vz;
+  float di, pi;
+  float fx, fy, fz;
+};
+
+__device__ __forceinline__ void force_pair(ForceI& I, const float4 a, const float4 v) {
+  const float dx = I.px - a.x, dy = I.py - a.y, dz = I.pz - a.z;
+  const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  const float rr = sqrtf(dsq * FParams.d2);
+  const float c = FParams.psmoothradius - rr;
+  const float dj = a.w;
+  const float pj = (dj - FParams.prest_dens) * FParams.pintstiff;
+  const float pterm = (FParams.sim_scale * -0.5f) * c * FParams.spikykern * (I.pi + pj) / rr;
+  const float inv = 1.0f / (I.di * dj);
+  const float vterm = FParams.vterm;
+  const float sx = fmaf(pterm, dx, vterm * (v.x - I.vx));
+  const float sy = fmaf(pterm, dy, vterm * (v.y - I.vy));
+  const float sz = fmaf(pterm, dz, vterm * (v.z - I.vz));
+  I.fx = fmaf(sx * c, inv, I.fx);
+  I.fy = fmaf(sy * c, inv, I.fy);
+  I.fz = fmaf(sz * c, inv, I.fz);
+}
+
+__device__ __forceinline__ bool is_pair(const ForceI& I, const float4 a) {
+  const float dx = I.px - a.x, dy = I.py - a.y, dz = I.pz - a.z;
+  const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  return dsq < FParams.rd2 && dsq > 0.0f;
+}
+
+extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
+  __shared__ uint fq[FQ_SLOTS];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + t;
   const uint* __restrict__ off = pbuf<const uint>(FAccel, AGRIDOFF);
   const uint* __restrict__ cnt = pbuf<const uint>(FAccel, AGRIDCNT);
-  const fx_float3 p = pos[i];
-  const fx_float3 vi = vev[i];
-  const float di = rho[i];
-  const float rest = FParams.prest_dens, stiff = FParams.pintstiff;
-  const float pi = (di - rest) * stiff;
-  const float h = FParams.psmoothradius, rd2 = FParams.rd2, d2 = FParams.d2;
-  const float ssh = FParams.sim_scale * -0.5f;
-  const float spiky = FParams.spikykern, vterm = FParams.vterm;
-  const int base = gc - (FParams.gridRes.z + 1) * FParams.gridRes.x - 1;
+  const uint* __restrict__ bits = pbuf<const uint>(FAccel, ANBRBITS);
+  const float4* __restrict__ fr = pbuf<const float4>(FAccel, AFREC);
   const int srch = FParams.gridSrch;
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  for (int r = 0; r < srch * srch; r++) {
-    int jb, je;
-    row_range(base, r, srch, off, cnt, jb, je);
-    for (int j = jb; j < je; j++) {
-      const fx_float3 q = pos[j];
-      const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-      const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      if (dsq < rd2 && dsq > 0.0f) {
-        const float rr = sqrtf(dsq * d2);
-        const float c = h - rr;
-        const float dj = rho[j];
-        const float pj = (dj - rest) * stiff;
-        const float pterm = ssh * c * spiky * (pi + pj) / rr;
-        const float inv = 1.0f / (di * dj);
-        const fx_float3 vj = vev[j];
-        const float sx = fmaf(pterm, dx, vterm * (vj.x - vi.x));
-        const float sy = fmaf(pterm, dy, vterm * (vj.y - vi.y));
-        const float sz = fmaf(pterm, dz, vterm * (vj.z - vi.z));
-        fx = fmaf(sx * c, inv, fx);
-        fy = fmaf(sy * c, inv, fy);
-        fz = fmaf(sz * c, inv, fz);
+  int gc = FX_GRID_UNDEF;
+  if (i < pnum) gc = pbuf<const int>(FPnts, FGCELL)[i];
+  const bool act = gc != FX_GRID_UNDEF;
+  const int base = gc - (FParams.gridRes.z + 1) * FParams.gridRes.x - 1;
+  ForceI I;
+  I.px = I.py = I.pz = I.vx = I.vy = I.vz = 0.0f;
+  I.di = 1.0f;
+  I.pi = 0.0f;
+  I.fx = I.fy = I.fz = 0.0f;
+
+  if (bits == nullptr || fr == nullptr) {
+    // no density-stage records (a host without ANBRBITS/AFREC): test every
+    // candidate and evaluate pairs in place
+    if (act) {
+      const fx_float3* __restrict__ pos = pbuf<const fx_float3>(FPnts, FPOS);
+      const fx_float3* __restrict__ vev = pbuf<const fx_float3>(FPnts, FVEVAL);
+      const float* __restrict__ rho = pbuf<const float>(FPnts, FPRESS);
+      const fx_float3 p = pos[i], v = vev[i];
+      I.px = p.x; I.py = p.y; I.pz = p.z;
+      I.vx = v.x; I.vy = v.y; I.vz = v.z;
+      I.di = rho[i];
+      I.pi = (I.di - FParams.prest_dens) * FParams.pintstiff;
+      for (int r = 0; r < srch * srch; r++) {
+        int jb, je;
+        row_range(base, r, srch, off, cnt, jb, je);
+        for (int j = jb; j < je; j++) {
+          const fx_float3 q = pos[j], u = vev[j];
+          const float4 a = make_float4(q.x, q.y, q.z, rho[j]);
+          if (is_pair(I, a)) force_pair(I, a, make_float4(u.x, u.y, u.z, 0.0f));
+        }
+      }
+      pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{I.fx, I.fy, I.fz};
+    }
+    return;
+  }
+
+  if (act) {
+    const float4 a = fr[2 * (size_t)i], v = fr[2 * (size_t)i + 1];
+    I.px = a.x; I.py = a.y; I.pz = a.z;
+    I.vx = v.x; I.vy = v.y; I.vz = v.z;
+    I.di = a.w;
+    I.pi = (I.di - FParams.prest_dens) * FParams.pintstiff;
+  }
+  // a particle without recorded bits (too many candidates) tests its own
+  const bool ovf = act && nbr_words(gc, off, cnt) > FX_NBR_WORDS;
+  if (ovf) {
+    for (int r = 0; r < srch * srch; r++) {
+      int jb, je;
+      row_range(base, r, srch, off, cnt, jb, je);
+      for (int j = jb; j < je; j++) {
+        const float4 a = fr[2 * (size_t)j];
+        if (is_pair(I, a)) force_pair(I, a, fr[2 * (size_t)j + 1]);
       }
     }
   }
-  pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{fx, fy, fz};
+  const bool use = act && !ovf;
+  const int fqn = FQ_SLOTS / (int)blockDim.x;  // ring entries per lane
+  if (fqn < 16) {
+    // blocks too large for a ring that holds a half-word per lane: evaluate
+    // each lane's pairs in place
+    int W = 0;
+    for (int r = 0; r < srch * srch; r++) {
+      int jb = 0, je = 0;
+      if (use) row_range(base, r, srch, off, cnt, jb, je);
+      const int nw = (je - jb + 31) >> 5;
+      for (int w = 0; w < nw; w++) {
+        uint m = bits[(size_t)(W + w) * pnum + i];
+        while (m != 0u) {
+          const uint j = (uint)(jb + (w << 5) + __builtin_ctz(m));
+          m &= m - 1u;
+          force_pair(I, fr[2 * (size_t)j], fr[2 * (size_t)j + 1]);
+        }
+      }
+      W += nw;
+    }
+    if (act) pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{I.fx, I.fy, I.fz};
+    return;
+  }
+  const size_t N = (size_t)pnum;
+  int head = 0, qn = 0;  // ring holds entries head .. head+qn-1 (mod fqn)
+  int W = 0;
+  // evaluates the ring's first f entries of every lane (fewer if it holds
+  // fewer); the loads are unconditional (record 0 stands in for an empty
+  // slot) so that the loop is branch-free and the next entry's loads can be
+  // issued ahead of the current entry's arithmetic
+  auto drain = [&](int f) {
+#pragma unroll 2
+    for (int e = 0; e < f; e++) {
+      const bool ok = e < qn;
+      const uint j = ok ? fq[head * (int)blockDim.x + t] : 0u;
+      const float4 a = fr[2 * (size_t)j], v = fr[2 * (size_t)j + 1];
+      ForceI J = I;
+      force_pair(J, a, v);
+      if (ok) {
+        I.fx = J.fx;
+        I.fy = J.fy;
+        I.fz = J.fz;
+        head = (head + 1 == fqn) ? 0 : head + 1;
+      }
+    }
+    qn = max(qn - f, 0);
+  };
+  for (int r = 0; r < srch * srch; r++) {
+    int jb = 0, je = 0;
+    if (use) row_range(base, r, srch, off, cnt, jb, je);
+    const int nw = (je - jb + 31) >> 5;
+    const int maxw = min(wave_max(nw), FX_NBR_WORDS);
+    for (int w = 0; w < maxw; w++) {
+      uint word = 0;
+      if (w < nw) word = bits[(size_t)(W + w) * N + i];
+      for (int h = 0; h < 32; h += 16) {  // a word in two halves of 16 candidates
+        uint m = (word >> h) & 0xffffu;
+        const int pc = __popc(m);
+        // make room: every lane must fit this half-word's pairs
+        if (__any(qn + pc > fqn)) {
+          const int need = wave_max(qn + pc - fqn);
+          drain(min(max(need, wave_min(use ? qn : 0x7fffffff)), fqn));
+        }
+        int tail = head + qn;
+        if (tail >= fqn) tail -= fqn;
+        while (m != 0u) {
+          const int b = __builtin_ctz(m);
+          m &= m - 1u;
+          fq[tail * (int)blockDim.x + t] = (uint)(jb + (w << 5) + h + b);
+          tail = (tail + 1 == fqn) ? 0 : tail + 1;
+          qn++;
+        }
+      }
+    }
+    W += nw;
+  }
+  drain(min(wave_max(qn), fqn));
+  if (act) pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{I.fx, I.fy, I.fz};
 }
 
 // ------------------------------------------------------ neighbour statistics

@@ -148,8 +148,10 @@ void update_params(FParams_t& P) {
 }
 
 void set_geometry(FParams_t& P, int threads) {
-  // computeNumBlocks (particles.cpp:26-30) for pnum particles
-  P.numThreads = threads < P.pnum ? threads : P.pnum;
+  // computeNumBlocks (particles.cpp:26-30) for pnum particles, with blocks
+  // of whole wave64s
+  const int waves = (P.pnum + 63) / 64 * 64;
+  P.numThreads = threads < waves ? threads : waves;
   P.numBlocks = P.numThreads == 0 ? 1 : (P.pnum + P.numThreads - 1) / P.numThreads;
   P.szPnts = P.numBlocks * P.numThreads;
 }

@@ -47,6 +47,8 @@ SIGNATURES = {
     "fx_bench_steps": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "fx_count_pairs": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "fx_set_block_size": (C.c_int, [_P, C.c_int32]),
+    "fx_set_stage_block_size": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "fx_stage_block_size": (C.c_int32, [_P, C.c_int32]),
     "fx_last_error": (C.c_char_p, []),
     "fx_version": (C.c_char_p, []),
 }
@@ -146,6 +148,12 @@ class Particles:
 
     def SetBlockSize(self, threads):
         check(lib().fx_set_block_size(self._h, threads), "fx_set_block_size")
+
+    def SetStageBlockSize(self, stage, threads):
+        check(lib().fx_set_stage_block_size(self._h, stage, threads), "fx_set_stage_block_size")
+
+    def StageBlockSize(self, stage):
+        return int(lib().fx_stage_block_size(self._h, stage))
 
     # ---- stages (particles.cpp:382-394)
     def InsertParticles(self):

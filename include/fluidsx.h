@@ -156,8 +156,10 @@ int fx_bench_steps(fx_system* sys, int32_t nsteps, int32_t use_graph, double* ms
 int fx_count_pairs(fx_system* sys, int64_t* candidates, int64_t* pairs);
 
 /* Launch geometry: threads per block of the per-particle kernels (64..1024,
- * multiple of 64). */
+ * multiple of 64), for every stage or for one.  Results do not depend on it. */
 int fx_set_block_size(fx_system* sys, int32_t threads);
+int fx_set_stage_block_size(fx_system* sys, int32_t stage, int32_t threads);
+int32_t fx_stage_block_size(const fx_system* sys, int32_t stage);
 
 /* Last error message of the calling thread. */
 const char* fx_last_error(void);

@@ -16,6 +16,8 @@
  * leaves unused):
  *   - FID = 8: a uint32 particle id carried through the counting sort so that
  *     results can be compared per particle.
+ *   - ANBRBITS, AFREC: neighbour bitmask and packed per-particle record that
+ *     the density stage leaves for the force stage.
  *   - AGRIDCNT / AGRIDOFF hold gridTotal+1 entries.  Slot gridTotal counts the
  *     particles whose cell is GRID_UNDEF, so the scan places them in a
  *     deterministic tail after every valid cell instead of leaving stale data
@@ -53,7 +55,12 @@ extern "C" {
 #define AAUXSCAN1 4
 #define AAUXARRAY2 5
 #define AAUXSCAN2 6
-#define FX_NUM_ABUF 7
+/* additions, written by the density stage for the force stage (optional: a
+ * host that does not allocate them gets the force stage without them) */
+#define ANBRBITS 7 /* neighbour bitmask: word w of particle i at [w * N + i] */
+#define AFREC 8    /* force record: 2 float4 per particle, (pos, rho), (veval, 0) */
+#define FX_NUM_ABUF 9
+#define FX_NBR_WORDS 128 /* bitmask words per particle (4096 candidates) */
 
 #define FX_DMAXBUF 16 /* datax.h:37 */
 

@@ -164,16 +164,21 @@ def test_graph_equals_eager(ps):
 
 @pytest.mark.parametrize("threads", [64, 128, 512, 1024])
 def test_block_size_invariance(ps, threads):
+    """Results do not depend on the launch geometry (the force stage switches
+    from its pair ring to in-place evaluation above 64 threads per block)."""
     sc = scenes.cfg0()
-    ps.SetBlockSize(256)
+    defaults = [ps.StageBlockSize(k) for k in range(abi.NUM_STAGES)]
     ps.Restart(sc)
     ps.Run(3)
     base = ps.state()
     ps.SetBlockSize(threads)
-    ps.Restart(sc)
-    ps.Run(3)
-    g = ps.state()
-    ps.SetBlockSize(256)
+    try:
+        ps.Restart(sc)
+        ps.Run(3)
+        g = ps.state()
+    finally:
+        for k, t in enumerate(defaults):
+            ps.SetStageBlockSize(k, t)
     for k in ("pos", "vel", "veval", "press", "force"):
         assert bits_equal(base[k], g[k]), (threads, k)
 
