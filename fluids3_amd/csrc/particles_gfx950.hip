@@ -62,6 +62,24 @@ __device__ __forceinline__ uint wave_inclusive_scan(uint v) {
   return v;
 }
 
+// XCD-aware block order.  MI355X hands consecutive workgroups to its eight
+// XCDs round-robin, and each XCD has its own L2.  Cell-sorted particles that
+// are close in index share neighbours, so runs of FX_XCD_RUN consecutive
+// logical blocks are given to one XCD, and the runs rotate over the XCDs (a
+// contiguous share per XCD would unbalance the load, since the work per
+// particle follows the local density).  Dispatch slot b is the k-th block
+// of XCD x = b % 8; it runs logical block (run * 8 + x) * FX_XCD_RUN + k % RUN
+// with run = k / RUN.  Blocks past the last full round keep their index.
+#define FX_NUM_XCD 8
+#define FX_XCD_RUN 32
+__device__ __forceinline__ int xcd_block() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int round = FX_NUM_XCD * FX_XCD_RUN;
+  if (b >= nb / round * round) return b;
+  const int x = b % FX_NUM_XCD, k = b / FX_NUM_XCD;
+  return ((k / FX_XCD_RUN) * FX_NUM_XCD + x) * FX_XCD_RUN + k % FX_XCD_RUN;
+}
+
 // ----------------------------------------------------------- fill (memset)
 
 // Fill used instead of hipMemsetAsync so that a captured step graph holds
@@ -408,6 +426,12 @@ struct ForceI {
   float fx, fy, fz;
 };
 
+// One pair's term, arranged around a single correctly rounded reciprocal:
+// with D = rho_i rho_j and inv = 1 / (r D),
+//   F += A dist + B (veval_j - veval_i),
+//   A = ((ss * -0.5) * spiky) c^2 (p_i + p_j) inv,  B = vterm c r inv,
+// which is the reference's (pterm dist + vterm dv) c / D with
+// pterm = ss (-0.5) c spiky (p_i + p_j) / r.
 __device__ __forceinline__ void force_pair(ForceI& I, const float4 a, const float4 v) {
   const float dx = I.px - a.x, dy = I.py - a.y, dz = I.pz - a.z;
   const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
@@ -415,15 +439,13 @@ __device__ __forceinline__ void force_pair(ForceI& I, const float4 a, const floa
   const float c = FParams.psmoothradius - rr;
   const float dj = a.w;
   const float pj = (dj - FParams.prest_dens) * FParams.pintstiff;
-  const float pterm = (FParams.sim_scale * -0.5f) * c * FParams.spikykern * (I.pi + pj) / rr;
-  const float inv = 1.0f / (I.di * dj);
-  const float vterm = FParams.vterm;
-  const float sx = fmaf(pterm, dx, vterm * (v.x - I.vx));
-  const float sy = fmaf(pterm, dy, vterm * (v.y - I.vy));
-  const float sz = fmaf(pterm, dz, vterm * (v.z - I.vz));
-  I.fx = fmaf(sx * c, inv, I.fx);
-  I.fy = fmaf(sy * c, inv, I.fy);
-  I.fz = fmaf(sz * c, inv, I.fz);
+  const float inv = 1.0f / (rr * (I.di * dj));
+  const float kp = (FParams.sim_scale * -0.5f) * FParams.spikykern;
+  const float A = (kp * (c * c)) * (I.pi + pj) * inv;
+  const float B = (FParams.vterm * c) * rr * inv;
+  I.fx = fmaf(A, dx, fmaf(B, v.x - I.vx, I.fx));
+  I.fy = fmaf(A, dy, fmaf(B, v.y - I.vy, I.fy));
+  I.fz = fmaf(A, dz, fmaf(B, v.z - I.vz, I.fz));
 }
 
 __device__ __forceinline__ bool is_pair(const ForceI& I, const float4 a) {
@@ -435,7 +457,7 @@ __device__ __forceinline__ bool is_pair(const ForceI& I, const float4 a) {
 extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
   __shared__ uint fq[FQ_SLOTS];
   const int t = threadIdx.x;
-  const int i = blockIdx.x * blockDim.x + t;
+  const int i = xcd_block() * blockDim.x + t;
   const uint* __restrict__ off = pbuf<const uint>(FAccel, AGRIDOFF);
   const uint* __restrict__ cnt = pbuf<const uint>(FAccel, AGRIDCNT);
   const uint* __restrict__ bits = pbuf<const uint>(FAccel, ANBRBITS);

@@ -162,8 +162,12 @@ void pressure_one(const FParams_t& P, State<R>& S, int i) {
 }
 
 // --------------------------------------------------------------- force
-// particles.cu:188-225 (fluids3.9 fluid_system.cpp:573-631); the float3
-// division of cutil_math is a multiplication by the reciprocal.
+// particles.cu:188-225 (fluids3.9 fluid_system.cpp:573-631):
+//   F_i += (pterm dist + vterm (veval_j - veval_i)) c / (rho_i rho_j),
+//   pterm = ss (-0.5) c spiky (p_i + p_j) / r,  c = h - r,  p = (rho - rho0) k.
+// Evaluated with one reciprocal: inv = 1 / (r rho_i rho_j),
+//   A = ((ss -0.5) spiky) c^2 (p_i + p_j) inv,  B = vterm c r inv,
+//   F += A dist + B (veval_j - veval_i).
 template <class R>
 void force_one(const FParams_t& P, State<R>& S, int i) {
   const int gc = S.gcell[i];
@@ -171,8 +175,8 @@ void force_one(const FParams_t& P, State<R>& S, int i) {
   const int base = gc - (P.gridRes.z + 1) * P.gridRes.x - 1;
   const R rd2 = P.rd2, d2 = P.d2, h = P.psmoothradius;
   const R rest = P.prest_dens, stiff = P.pintstiff;
-  const R ssh = (R)P.sim_scale * R(-0.5);
-  const R spiky = P.spikykern, vterm = P.vterm;
+  const R kp = ((R)P.sim_scale * R(-0.5)) * (R)P.spikykern;
+  const R vterm = P.vterm;
   const V3<R> p = S.pos[i];
   const V3<R> vi = S.veval[i];
   const R di = S.press[i];
@@ -190,15 +194,13 @@ void force_one(const FParams_t& P, State<R>& S, int i) {
         const R c = h - rr;
         const R dj = S.press[j];
         const R pj = (dj - rest) * stiff;
-        const R pterm = ssh * c * spiky * (pi + pj) / rr;
-        const R inv = R(1) / (di * dj);
+        const R inv = R(1) / (rr * (di * dj));
+        const R A = (kp * (c * c)) * (pi + pj) * inv;
+        const R B = (vterm * c) * rr * inv;
         const V3<R> vj = S.veval[j];
-        const R sx = std::fma(pterm, dx, vterm * (vj.x - vi.x));
-        const R sy = std::fma(pterm, dy, vterm * (vj.y - vi.y));
-        const R sz = std::fma(pterm, dz, vterm * (vj.z - vi.z));
-        fx = std::fma(sx * c, inv, fx);
-        fy = std::fma(sy * c, inv, fy);
-        fz = std::fma(sz * c, inv, fz);
+        fx = std::fma(A, dx, std::fma(B, vj.x - vi.x, fx));
+        fy = std::fma(A, dy, std::fma(B, vj.y - vi.y, fy));
+        fz = std::fma(A, dz, std::fma(B, vj.z - vi.z, fz));
       }
     }
   }
