@@ -359,7 +359,8 @@ extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
       jb = (int)off[gc + adj];
       je = (int)(off[gc + adj + srch - 1] + cnt[gc + adj + srch - 1]);
     }
-    uint acc = 0;
+    uint acc = 0, bit = 1u;  // bits of the current word, and the next one's mask
+    int wcur = W;            // current word
     for (int c0 = lo; c0 < hi; c0 += TILE_P) {
       const int n = min(TILE_P, hi - c0);
       for (int k = t; k < n; k += blockDim.x) {
@@ -378,16 +379,18 @@ extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
           sum = fmaf(tt * tt, tt, sum);
         }
         if (rec) {
-          const int bp = c0 + k - jb;  // bit position in the row window
-          acc |= (pair ? 1u : 0u) << (bp & 31);
-          if ((bp & 31) == 31 || c0 + k == je - 1) {
-            bits[(size_t)(W + (bp >> 5)) * N + i] = acc;
+          acc |= pair ? bit : 0u;
+          if (bit == 0x80000000u) {  // word complete
+            bits[(size_t)wcur * N + i] = acc;
             acc = 0;
+            wcur++;
           }
+          bit = __builtin_amdgcn_alignbit(bit, bit, 31);  // rotate left by one
         }
       }
       __syncthreads();
     }
+    if (rec && bit != 1u) bits[(size_t)wcur * N + i] = acc;  // the row's partial last word
     W += (je - jb + 31) >> 5;
   }
   if (act) {
