@@ -43,6 +43,10 @@ def flops_per_particle(stage, C, P):
 
 
 def describe(name, sc, N, G):
+    if isinstance(sc, list):
+        n = [s.num_points for s in sc]
+        return (f"{name}: {len(sc)} independent dam breaks in one batch, {N:,} particles "
+                f"({min(n):,}..{max(n):,} per scene), {G:,} grid cells, one launch per stage")
     b0, b1 = sc.bound_min.tolist(), sc.bound_max.tolist()
     i0, i1 = sc.init_min.tolist(), sc.init_max.tolist()
     r = lambda v: "(" + ",".join(f"{x:g}" for x in v) + ")"
@@ -53,9 +57,9 @@ def describe(name, sc, N, G):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (200; 100 for cfg2)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg1_rest"])
+    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg1_rest", "cfg2"])
     ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly")
     ap.add_argument("--stats-every", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -85,11 +89,20 @@ def main():
 
     from fluids3_amd import Particles, scenes
 
-    sc = scenes.PRESETS[args.workload]()
-    K, W = args.steps, args.warmup
+    batch = args.workload == "cfg2"
+    K = args.steps if args.steps is not None else (100 if batch else 200)
+    W = args.warmup
     ps = Particles()
-    ps.Restart(sc)
-    N, G = ps.NumPoints(), ps.GridTotal()
+    if batch:
+        # 64 independent dam breaks, counts log-uniform in [8192, 262144],
+        # viscosity uniform in [0.2, 0.5], each box filled to 0.4
+        sc = scenes.batch_scenes(64)
+        restart = lambda: ps.RestartBatch(sc)
+    else:
+        sc = scenes.PRESETS[args.workload]()
+        restart = lambda: ps.Restart(sc)
+    restart()
+    N, G = ps.NumPoints(), ps.CellTableSize() - ps.NumScenes()
     p = ps.params
 
     # warm-up, then the timed region (one graph of K steps with per-stage events;
@@ -104,7 +117,7 @@ def main():
             raise
         print(f"graph-timed region unavailable ({e}); timing eager launches", file=sys.stderr)
         use_graph = False
-        ps.Restart(sc)
+        restart()
         ps.Run(W)
         ps.Synchronize()
         ms_total, stage_ms = ps.BenchSteps(K, use_graph=False)
@@ -112,9 +125,10 @@ def main():
     ms_step = ms_total / K
 
     # same trajectory again (bit-reproducible), untimed, for C and P
-    ps.Restart(sc)
+    restart()
     ps.Run(W)
-    start_state = ps.state() if not args.no_cpu_baseline else None
+    want_cpu = not args.no_cpu_baseline and not batch
+    start_state = ps.state() if want_cpu else None
     csum = psum = 0
     samples = 0
     for s in range(K):
@@ -160,7 +174,7 @@ def main():
     roof["traffic"] = None
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if want_cpu:
         from oracle import oracle
 
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
@@ -173,7 +187,8 @@ def main():
                          f"of the timed window on {threads} host threads ({sec:.2f} s)"}
 
     out = {
-        "metric": "particle-steps/sec at 4M particles (1 GPU)" if N == 4_000_000 else "particle-steps/sec",
+        "metric": ("particle-steps/sec at 4M particles (1 GPU)" if N == 4_000_000 and not batch
+                   else "particle-steps/sec"),
         "value": round(value, 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,

@@ -59,6 +59,27 @@ int fx_restart(fx_system* s, const fx_scene* scene) {
   return s->p.Restart(*scene);
 }
 
+int fx_restart_batch(fx_system* s, const fx_scene* scenes, int32_t count) {
+  FX_CHECK_SYS(s);
+  return s->p.RestartBatch(scenes, count);
+}
+
+int32_t fx_num_scenes(const fx_system* s) { return s == nullptr ? 0 : s->p.NumScenes(); }
+
+int fx_scene_range(const fx_system* s, int32_t scene, int32_t* first_particle, int32_t* num_particles,
+                   int32_t* first_cell, int32_t* num_cells) {
+  FX_CHECK_SYS(s);
+  if (!first_particle || !num_particles || !first_cell || !num_cells)
+    return fx::fail(FX_EINVAL, "fx_scene_range: null argument");
+  return s->p.SceneRange(scene, first_particle, num_particles, first_cell, num_cells);
+}
+
+int fx_scene_params(const fx_system* s, int32_t scene, FParams_t* out) {
+  FX_CHECK_SYS(s);
+  if (out == nullptr) return fx::fail(FX_EINVAL, "fx_scene_params: null argument");
+  return s->p.SceneParams(scene, out);
+}
+
 int fx_get_params(const fx_system* s, FParams_t* out) {
   FX_CHECK_SYS(s);
   if (out == nullptr) return fx::fail(FX_EINVAL, "fx_get_params: null argument");
@@ -74,6 +95,7 @@ int fx_set_params(fx_system* s, const FParams_t* p) {
 
 int32_t fx_num_points(const fx_system* s) { return s == nullptr ? 0 : s->p.NumPoints(); }
 int32_t fx_grid_total(const fx_system* s) { return s == nullptr ? 0 : s->p.Params().gridTotal; }
+int32_t fx_cell_table_size(const fx_system* s) { return s == nullptr ? 0 : s->p.CellTableSize(); }
 
 int32_t fx_buffer_stride(int32_t buf) {
   switch (buf) {

@@ -16,12 +16,26 @@
 static const char* kKernelNames[FUNC_MAX] = {
     "insertParticles",   "countingSortFull", "computePressure", "computeForce",
     "advanceParticles",  "prefixSum",        "prefixFixup",     "countingSortIndex",
-    "fxFillUint",        "advanceParticlesClock", "fxClockTick",     "fxCountPairs"};
+    "fxFillUint",        "advanceParticlesClock", "fxClockTick",     "fxCountPairs",
+    "insertParticlesBatch", "countingSortIndexBatch", "countingSortFullBatch", "computePressureBatch",
+    "computeForceBatch", "advanceParticlesBatch", "fxCountPairsBatch"};
 
 Particles::Particles() { memset(&m_Params, 0, sizeof(m_Params)); }
 
+static void free_batch(void*& a, void*& b, void*& c, std::map<int, std::pair<void*, int>>& t) {
+  for (void** p : {&a, &b, &c})
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  for (auto& e : t)
+    if (e.second.first) (void)hipFree(e.second.first);
+  t.clear();
+}
+
 Particles::~Particles() {
   ReleaseGraph();
+  free_batch(m_DParams, m_DPBase, m_DCBase, m_BTab);
   m_Points.DeleteAllBuffers();
   m_PointsTemp.DeleteAllBuffers();
   m_Accel.DeleteAllBuffers();
@@ -71,12 +85,18 @@ int Particles::Restart(const fx_scene& sc) {
   if (m_Module == nullptr) return fx::fail(FX_ESTATE, "Restart: call Initialize first");
   FX_HIP(hipStreamSynchronize(m_Stream), "Restart");
   ReleaseGraph();
+  free_batch(m_DParams, m_DPBase, m_DCBase, m_BTab);
+  m_Batch = false;
+  m_SceneParams.clear();
   const int threads = m_Params.numThreads > 0 ? m_Params.numThreads : 256;
   FParams_t P;
   FX_TRY(fx::scene_derive(sc, P));
   m_Params = P;
   mNumPoints = 0;
   mMaxPoints = sc.num_points;
+  m_ScanCells = m_Params.gridTotal + 1;
+  m_PBase = {0, sc.num_points};
+  m_CBase = {0, m_ScanCells};
   FX_TRY(ReallocateParticles(mMaxPoints));
   fx::set_geometry(m_Params, threads);
   FX_TRY(RebuildAccelGrid());
@@ -87,6 +107,106 @@ int Particles::Restart(const fx_scene& sc) {
   if (n < 0) return n;
   if (n != mMaxPoints)
     return fx::fail(FX_ERANGE, "Restart: lattice holds %d of %d particles", n, mMaxPoints);
+  return FinishRestart(n);
+}
+
+int Particles::RestartBatch(const fx_scene* scenes, int count) {
+  if (m_Module == nullptr) return fx::fail(FX_ESTATE, "RestartBatch: call Initialize first");
+  if (scenes == nullptr || count <= 0) return fx::fail(FX_EINVAL, "RestartBatch: no scenes");
+  FX_HIP(hipStreamSynchronize(m_Stream), "RestartBatch");
+  ReleaseGraph();
+  free_batch(m_DParams, m_DPBase, m_DCBase, m_BTab);
+  std::vector<FParams_t> ps(count);
+  std::vector<int> pb(count + 1, 0), cb(count + 1, 0);
+  for (int s = 0; s < count; s++) {
+    FX_TRY(fx::scene_derive(scenes[s], ps[s]));
+    if ((int64_t)pb[s] + scenes[s].num_points > INT32_MAX ||
+        (int64_t)cb[s] + ps[s].gridTotal + 1 > (int64_t)1024 * 1024 * 1024 - 1)
+      return fx::fail(FX_ERANGE, "RestartBatch: the batch exceeds 2^31 particles or 1024^3 cells");
+    pb[s + 1] = pb[s] + scenes[s].num_points;
+    cb[s + 1] = cb[s] + ps[s].gridTotal + 1;
+  }
+  const int threads = m_Params.numThreads > 0 ? m_Params.numThreads : 256;
+  m_Batch = true;
+  m_SceneParams = ps;
+  m_PBase = pb;
+  m_CBase = cb;
+  m_Params = ps[0];  // time, dt and the single-scene symbol
+  mNumPoints = 0;
+  mMaxPoints = pb[count];
+  m_ScanCells = cb[count];
+  m_Params.pnum = mMaxPoints;
+  FX_TRY(ReallocateParticles(mMaxPoints));
+  fx::set_geometry(m_Params, threads);
+  FX_TRY(RebuildAccelGrid());
+  for (int s = 0; s < count; s++) {
+    ps[s].pnum = scenes[s].num_points;
+    const int n = fx::scene_fill(scenes[s], reinterpret_cast<float*>(m_Points.bufF3(FPOS, pb[s])),
+                                 m_Points.bufUI(FCLR, pb[s]), scenes[s].num_points);
+    if (n < 0) return n;
+    if (n != scenes[s].num_points)
+      return fx::fail(FX_ERANGE, "RestartBatch: scene %d lattice holds %d of %d particles", s, n,
+                      scenes[s].num_points);
+  }
+  m_SceneParams = ps;
+  FX_HIP(hipMalloc(&m_DParams, sizeof(FParams_t) * count), "RestartBatch");
+  FX_HIP(hipMalloc(&m_DPBase, sizeof(int) * (count + 1)), "RestartBatch");
+  FX_HIP(hipMalloc(&m_DCBase, sizeof(int) * (count + 1)), "RestartBatch");
+  FX_HIP(hipMemcpyHtoD(m_DParams, ps.data(), sizeof(FParams_t) * count), "RestartBatch");
+  FX_HIP(hipMemcpyHtoD(m_DPBase, pb.data(), sizeof(int) * (count + 1)), "RestartBatch");
+  FX_HIP(hipMemcpyHtoD(m_DCBase, cb.data(), sizeof(int) * (count + 1)), "RestartBatch");
+  FX_TRY(BuildBlockTables());
+  return FinishRestart(mMaxPoints);
+}
+
+int Particles::BuildBlockTables() {
+  // (scene, block inside the scene) for every block of every stage's launch
+  for (auto& e : m_BTab)
+    if (e.second.first) (void)hipFree(e.second.first);
+  m_BTab.clear();
+  if (!m_Batch) return FX_OK;
+  for (int k = 0; k < FX_NUM_STAGES; k++) {
+    const int T = m_StageThreads[k];
+    if (m_BTab.count(T)) continue;
+    std::vector<int> tab;
+    for (int s = 0; s + 1 < (int)m_PBase.size(); s++) {
+      const int nb = (m_PBase[s + 1] - m_PBase[s] + T - 1) / T;
+      for (int b = 0; b < nb; b++) {
+        tab.push_back(s);
+        tab.push_back(b);
+      }
+    }
+    void* d = nullptr;
+    FX_HIP(hipMalloc(&d, tab.size() * sizeof(int) + 8), "BuildBlockTables");
+    FX_HIP(hipMemcpyHtoD(d, tab.data(), tab.size() * sizeof(int)), "BuildBlockTables");
+    m_BTab[T] = {d, (int)(tab.size() / 2)};
+  }
+  return FX_OK;
+}
+
+FxBatchArgs Particles::BatchFor(int stage, unsigned* blocks) const {
+  const auto it = m_BTab.find(m_StageThreads[stage]);
+  *blocks = it == m_BTab.end() ? 0u : (unsigned)it->second.second;
+  return FxBatchArgs{m_DParams, m_DPBase, m_DCBase, it == m_BTab.end() ? nullptr : it->second.first};
+}
+
+int Particles::SceneRange(int s, int* p0, int* np, int* c0, int* nc) const {
+  if (s < 0 || s + 1 >= (int)m_PBase.size()) return fx::fail(FX_EINVAL, "scene %d out of range", s);
+  *p0 = m_PBase[s];
+  *np = m_PBase[s + 1] - m_PBase[s];
+  *c0 = m_CBase[s];
+  *nc = m_CBase[s + 1] - m_CBase[s];
+  return FX_OK;
+}
+
+int Particles::SceneParams(int s, FParams_t* out) const {
+  if (s < 0 || s >= NumScenes()) return fx::fail(FX_EINVAL, "scene %d out of range", s);
+  *out = m_Batch ? m_SceneParams[s] : m_Params;
+  if (m_Batch) out->time = m_Params.time;
+  return FX_OK;
+}
+
+int Particles::FinishRestart(int n) {
   memset(m_Points.bufF3(FVEL), 0, sizeof(fx_float3) * (size_t)n);
   memset(m_Points.bufF3(FVEVAL), 0, sizeof(fx_float3) * (size_t)n);
   memset(m_Points.bufF3(FFORCE), 0, sizeof(fx_float3) * (size_t)n);
@@ -137,7 +257,7 @@ int Particles::RebuildAccelGrid() {
   // particles.cpp:548-626; the counts and offsets carry one extra slot
   // (index gridTotal) for out-of-range particles.
   FX_TRY(fx::rebuild_grid(m_Params));
-  const int numElem1 = m_Params.gridTotal + 1;
+  const int numElem1 = m_ScanCells;  // every scene's cells and tail bucket
   const int numElem2 = numElem1 / SCAN_ELEMS + 1;
   const int numElem3 = numElem2 / SCAN_ELEMS + 1;
   if (numElem3 > SCAN_ELEMS) return fx::fail(FX_ERANGE, "RebuildAccelGrid: %d cells exceed the scan", numElem1);
@@ -184,6 +304,7 @@ int Particles::SetParams(const FParams_t& p) {
       memcmp(&p.gridDelta, &q.gridDelta, sizeof(fx_float3)) != 0 || p.gridSrch != q.gridSrch ||
       p.pnum != q.pnum || p.psmoothradius != q.psmoothradius || p.sim_scale != q.sim_scale)
     return fx::fail(FX_ESTATE, "SetParams: the grid, smoothing radius, scale and count are fixed by Restart");
+  if (m_Batch) return fx::fail(FX_ESTATE, "SetParams: a batch takes its parameters from its scenes");
   ReleaseGraph();
   const int threads = m_Params.numThreads;
   m_Params = p;
@@ -198,7 +319,7 @@ int Particles::SetBlockSize(int threads) {
   m_Params.numThreads = threads;
   for (int k = 0; k < FX_NUM_STAGES; k++) m_StageThreads[k] = threads;
   if (mMaxPoints > 0) fx::set_geometry(m_Params, threads);
-  return FX_OK;
+  return BuildBlockTables();
 }
 
 int Particles::SetStageBlockSize(int stage, int threads) {
@@ -207,7 +328,7 @@ int Particles::SetStageBlockSize(int stage, int threads) {
     return fx::fail(FX_EINVAL, "SetStageBlockSize: %d is not a multiple of 64 in [64, 1024]", threads);
   ReleaseGraph();
   m_StageThreads[stage] = threads;
-  return FX_OK;
+  return BuildBlockTables();
 }
 
 int Particles::CommitAll() {
@@ -220,11 +341,17 @@ int Particles::CommitAll() {
 int Particles::InsertParticles() {
   // particles.cpp:716-730.  Only the counts need clearing: gcell and gndx of
   // every particle are written by the kernel.
-  int ncell = m_Params.gridTotal + 1;
+  int ncell = m_ScanCells;
   unsigned zero = 0;
   void* cnt = m_Accel.gpu(AGRIDCNT);
   void* fargs[] = {&cnt, &zero, &ncell};
   FX_TRY(Launch(FUNC_FILL, (unsigned)((ncell + 255) / 256), 256, fargs));
+  if (m_Batch) {
+    unsigned nb;
+    FxBatchArgs B = BatchFor(FX_STAGE_INSERT, &nb);
+    void* bargs[] = {&B};
+    return Launch(FUNC_INSERT_B, nb, (unsigned)m_StageThreads[FX_STAGE_INSERT], bargs);
+  }
   void* args[] = {&mNumPoints};
   return LaunchP(FUNC_INSERT, FX_STAGE_INSERT, args);
 }
@@ -233,7 +360,7 @@ int Particles::PrefixScanParticles() {
   // particles.cpp:774-815, over gridTotal+1 counts
   return ScanLevels(m_Accel.gpu(AGRIDCNT), m_Accel.gpu(AGRIDOFF), m_Accel.gpu(AAUXARRAY1),
                     m_Accel.gpu(AAUXSCAN1), m_Accel.gpu(AAUXARRAY2), m_Accel.gpu(AAUXSCAN2),
-                    m_Params.gridTotal + 1);
+                    m_ScanCells);
 }
 
 int Particles::ScanLevels(void* array1, void* scan1, void* array2, void* scan2, void* array3, void* scan3,
@@ -275,17 +402,36 @@ int Particles::CountingSort() {
   FX_TRY(m_Points.SwapGPU(&m_PointsTemp));
   m_Parity ^= 1;
   FX_TRY(PublishTables());
+  if (m_Batch) {
+    unsigned nb;
+    FxBatchArgs B = BatchFor(FX_STAGE_SORT, &nb);
+    void* bargs[] = {&B};
+    FX_TRY(Launch(FUNC_SORT_INDEX_B, nb, (unsigned)m_StageThreads[FX_STAGE_SORT], bargs));
+    return Launch(FUNC_SORT_FULL_B, nb, (unsigned)m_StageThreads[FX_STAGE_SORT], bargs);
+  }
   void* args[] = {&mNumPoints};
   FX_TRY(LaunchP(FUNC_SORT_INDEX, FX_STAGE_SORT, args));
   return LaunchP(FUNC_COUNTING_SORT, FX_STAGE_SORT, args);
 }
 
 int Particles::ComputePressure() {
+  if (m_Batch) {
+    unsigned nb;
+    FxBatchArgs B = BatchFor(FX_STAGE_PRESSURE, &nb);
+    void* bargs[] = {&B, &mNumPoints};
+    return Launch(FUNC_PRESS_B, nb, (unsigned)m_StageThreads[FX_STAGE_PRESSURE], bargs);
+  }
   void* args[] = {&mNumPoints};
   return LaunchP(FUNC_COMPUTE_PRESS, FX_STAGE_PRESSURE, args);
 }
 
 int Particles::ComputeForce() {
+  if (m_Batch) {
+    unsigned nb;
+    FxBatchArgs B = BatchFor(FX_STAGE_FORCE, &nb);
+    void* bargs[] = {&B, &mNumPoints};
+    return Launch(FUNC_FORCE_B, nb, (unsigned)m_StageThreads[FX_STAGE_FORCE], bargs);
+  }
   void* args[] = {&mNumPoints};
   return LaunchP(FUNC_COMPUTE_FORCE, FX_STAGE_FORCE, args);
 }
@@ -294,6 +440,17 @@ int Particles::Advance() {
   // particles.cpp:413-418 (launched over pnum, which equals mNumPoints here)
   float dt = m_Params.dt, ss = m_Params.sim_scale;
   int n = m_Params.pnum;
+  if (m_Batch) {
+    unsigned nb;
+    FxBatchArgs B = BatchFor(FX_STAGE_ADVANCE, &nb);
+    float time = m_Params.time;
+    void* clk = m_Capturing ? (void*)m_Clock : nullptr;
+    void* bargs[] = {&B, &clk, &time};
+    FX_TRY(Launch(FUNC_ADVANCE_B, nb, (unsigned)m_StageThreads[FX_STAGE_ADVANCE], bargs));
+    if (!m_Capturing) return FX_OK;
+    void* targs[] = {&m_Clock, &dt};
+    return Launch(FUNC_CLOCK_TICK, 1, 64, targs);
+  }
   if (m_Capturing) {
     void* args[] = {&m_Clock, &dt, &ss, &n};
     FX_TRY(LaunchP(FUNC_ADVANCE_CLOCK, FX_STAGE_ADVANCE, args));
@@ -490,8 +647,15 @@ int Particles::CountPairs(int64_t* cand, int64_t* pairs) {
   void* st = m_Stats;
   void* fargs[] = {&st, &zero, &four};
   FX_TRY(Launch(FUNC_FILL, 1, 64, fargs));
-  void* args[] = {&st, &mNumPoints};
-  FX_TRY(LaunchP(FUNC_COUNT_PAIRS, FX_STAGE_PRESSURE, args));
+  if (m_Batch) {
+    unsigned nb;
+    FxBatchArgs B = BatchFor(FX_STAGE_PRESSURE, &nb);
+    void* bargs[] = {&B, &st};
+    FX_TRY(Launch(FUNC_COUNT_PAIRS_B, nb, (unsigned)m_StageThreads[FX_STAGE_PRESSURE], bargs));
+  } else {
+    void* args[] = {&st, &mNumPoints};
+    FX_TRY(LaunchP(FUNC_COUNT_PAIRS, FX_STAGE_PRESSURE, args));
+  }
   unsigned long long h[2] = {0, 0};
   FX_HIP(hipStreamSynchronize(m_Stream), "count_pairs");
   FX_HIP(hipMemcpyDtoH(h, m_Stats, sizeof(h)), "count_pairs");

@@ -30,7 +30,25 @@
 #define FUNC_ADVANCE_CLOCK 9
 #define FUNC_CLOCK_TICK 10
 #define FUNC_COUNT_PAIRS 11
-#define FUNC_MAX 12
+#define FUNC_INSERT_B 12
+#define FUNC_SORT_INDEX_B 13
+#define FUNC_SORT_FULL_B 14
+#define FUNC_PRESS_B 15
+#define FUNC_FORCE_B 16
+#define FUNC_ADVANCE_B 17
+#define FUNC_COUNT_PAIRS_B 18
+#define FUNC_MAX 19
+
+#include <map>
+#include <vector>
+
+// Kernel argument block of the *Batch kernels (BatchArgs in the kernel file).
+struct FxBatchArgs {
+  const void* params;
+  const void* pbase;
+  const void* cbase;
+  const void* btab;
+};
 
 class Particles {
  public:
@@ -42,6 +60,12 @@ class Particles {
   // set-up (particles.cpp:68-119, 267-300, 548-679)
   int Initialize(const char* code_object_path);
   int Restart(const fx_scene& scene);
+  // Many independent scenes in one system: particles and cell tables are
+  // concatenated, every stage is one launch over all scenes.
+  int RestartBatch(const fx_scene* scenes, int count);
+  int NumScenes() const { return m_Batch ? (int)m_SceneParams.size() : 1; }
+  int SceneRange(int s, int* p0, int* np, int* c0, int* nc) const;
+  int SceneParams(int s, FParams_t* out) const;
   int ReallocateParticles(int maxpnt);
   int RebuildAccelGrid();
   int UpdateParams();
@@ -81,6 +105,7 @@ class Particles {
   int CountPairs(int64_t* cand, int64_t* pairs);
 
   int NumPoints() const { return mNumPoints; }
+  int CellTableSize() const { return m_ScanCells; }
   const FParams_t& Params() const { return m_Params; }
   hipStream_t Stream() const { return m_Stream; }
 
@@ -91,6 +116,9 @@ class Particles {
   int ScanLevels(void* array1, void* scan1, void* array2, void* scan2, void* array3, void* scan3, int n);
   int PublishTables();
   void ReleaseGraph();
+  int BuildBlockTables();
+  FxBatchArgs BatchFor(int stage, unsigned* blocks) const;
+  int FinishRestart(int n);
 
   hipModule_t m_Module = nullptr;
   hipFunction_t m_Func[FUNC_MAX] = {};
@@ -118,4 +146,14 @@ class Particles {
   // threads per block of each stage's per-particle kernels; the force stage
   // runs one independent wave per block
   int m_StageThreads[FX_NUM_STAGES] = {256, 256, 256, 256, 64, 256};
+
+  // batch of scenes (m_Batch); a single scene uses the reference-named kernels
+  bool m_Batch = false;
+  int m_ScanCells = 0;  // entries of the cell tables: gridTotal+1, or the sum over scenes
+  std::vector<FParams_t> m_SceneParams;
+  std::vector<int> m_PBase, m_CBase;  // nscene+1 entries each
+  void* m_DParams = nullptr;          // FParams_t per scene
+  void* m_DPBase = nullptr;
+  void* m_DCBase = nullptr;
+  std::map<int, std::pair<void*, int>> m_BTab;  // threads per block -> (int2 table, blocks)
 };

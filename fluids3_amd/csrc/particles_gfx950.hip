@@ -5,6 +5,12 @@
 // (fluids5.0/src/particles.cu:17-20), so a host that loads kernels and
 // globals by name can drive it.
 //
+// Every stage is a device function over one scene: its FParams_t and its
+// particle and cell ranges.  The reference-named kernels run it for the
+// single scene held in FParams; the *Batch kernels run it for many
+// independent scenes concatenated in the same buffers, one launch per stage,
+// each block working inside one scene (a table maps block -> scene).
+//
 // Arithmetic contract (shared with the CPU oracle, which evaluates the same
 // expression trees): the file is compiled with -ffp-contract=off, every fused
 // multiply-add is an explicit fmaf(), sqrtf and '/' are IEEE correctly
@@ -38,8 +44,31 @@ __device__ __forceinline__ T* pbuf(const cuDataX& d, int id) {
   return (T*)(gT*)(d.mbuf[id]);
 }
 
-__device__ __forceinline__ fx_float3 ld3(const fx_float3* __restrict__ p, int i) {
-  return p[i];
+// A scene's parameters read through the constant address space, so that the
+// (block-uniform) reads are scalar loads.
+__device__ __forceinline__ const FParams_t* cparams(const FParams_t* p) {
+  typedef const __attribute__((address_space(4))) FParams_t cP;
+  return (const FParams_t*)(cP*)p;
+}
+
+// One block's view of its scene: particles [p0, p1), first cell c0 of the
+// scene's cell table, and the block's index inside the scene.
+struct Span {
+  int p0, p1, c0, blk;
+};
+
+// Batch lookup: entry b of the block table is (scene, block inside scene).
+struct BatchArgs {
+  const FParams_t* params;  // one FParams_t per scene
+  const int* pbase;         // first particle of each scene (nscene + 1 entries)
+  const int* cbase;         // first cell of each scene's table (nscene + 1)
+  const int2* btab;         // (scene, local block) per launched block
+};
+
+__device__ __forceinline__ Span batch_span(const BatchArgs& B, int b, const FParams_t*& P) {
+  const int2 e = B.btab[b];
+  P = cparams(B.params + e.x);
+  return Span{B.pbase[e.x], B.pbase[e.x + 1], B.cbase[e.x], e.y};
 }
 
 // Inclusive scan of one uint per lane across a wave64 with DPP row shifts and
@@ -60,6 +89,31 @@ __device__ __forceinline__ uint wave_inclusive_scan(uint v) {
   t = __builtin_amdgcn_update_dpp(0u, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
   v += t;
   return v;
+}
+
+// Wave-wide max / min with the same DPP pattern; lane 63 ends up with the
+// reduction of all 64 lanes and is read back as a scalar.
+template <bool kMax>
+__device__ __forceinline__ int wave_reduce(int v) {
+  const int id = kMax ? INT_MIN : INT_MAX;
+  int t;
+#define FX_RED_STEP(ctrl, rmask)                                    \
+  t = __builtin_amdgcn_update_dpp(id, v, ctrl, rmask, 0xf, false); \
+  v = kMax ? max(v, t) : min(v, t);
+  FX_RED_STEP(0x111, 0xf)  // row_shr:1
+  FX_RED_STEP(0x112, 0xf)  // row_shr:2
+  FX_RED_STEP(0x114, 0xf)  // row_shr:4
+  FX_RED_STEP(0x118, 0xf)  // row_shr:8
+  FX_RED_STEP(0x142, 0xa)  // row_bcast:15
+  FX_RED_STEP(0x143, 0xc)  // row_bcast:31
+#undef FX_RED_STEP
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_max(int v) { return wave_reduce<true>(v); }
+__device__ __forceinline__ int wave_min(int v) { return wave_reduce<false>(v); }
+
+__device__ __forceinline__ uint lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint)mask, 0u));
 }
 
 // XCD-aware block order.  MI355X hands consecutive workgroups to its eight
@@ -97,33 +151,29 @@ extern "C" __global__ void fxFillUint(uint* __restrict__ p, uint v, int n) {
 // The range test is done on the float before truncation (trunc(g) >= 1 iff
 // g >= 1; trunc(g) <= M iff g < M+1), which is the same predicate for every
 // finite input and sends NaN to GRID_UNDEF.  Out-of-range particles are
-// counted in slot gridTotal (the tail bucket).
+// counted in slot gridTotal of the scene (its tail bucket).  Stored cell
+// indices are offsets into the concatenated cell table (c0 + local cell).
 //
 // The counting is aggregated per wave: the particles arrive sorted by cell
 // from the previous step, so the 64 lanes of a wave fall into a few cells;
 // lanes of one cell take a single atomicAdd of their number and their slot
 // is the returned base plus their rank among those lanes.
-__device__ __forceinline__ uint lanes_below(unsigned long long mask) {
-  return __builtin_amdgcn_mbcnt_hi((uint)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint)mask, 0u));
-}
-
-extern "C" __global__ void insertParticles(int pnum) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = i < pnum;
+__device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S) {
+  const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
+  const bool active = i < S.p1;
   int gs = 0;
   if (active) {
-    const fx_float3 p = ld3(pbuf<const fx_float3>(FPnts, FPOS), i);
-    const float gx = (p.x - FParams.gridMin.x) * FParams.gridDelta.x;
-    const float gy = (p.y - FParams.gridMin.y) * FParams.gridDelta.y;
-    const float gz = (p.z - FParams.gridMin.z) * FParams.gridDelta.z;
-    const bool in = gx >= 1.0f && gx < (float)(FParams.gridScanMax.x + 1) &&
-                    gy >= 1.0f && gy < (float)(FParams.gridScanMax.y + 1) &&
-                    gz >= 1.0f && gz < (float)(FParams.gridScanMax.z + 1);
+    const fx_float3 p = pbuf<const fx_float3>(FPnts, FPOS)[i];
+    const float gx = (p.x - P.gridMin.x) * P.gridDelta.x;
+    const float gy = (p.y - P.gridMin.y) * P.gridDelta.y;
+    const float gz = (p.z - P.gridMin.z) * P.gridDelta.z;
+    const bool in = gx >= 1.0f && gx < (float)(P.gridScanMax.x + 1) && gy >= 1.0f &&
+                    gy < (float)(P.gridScanMax.y + 1) && gz >= 1.0f && gz < (float)(P.gridScanMax.z + 1);
     if (in) {
-      gs = ((int)gy * FParams.gridRes.z + (int)gz) * FParams.gridRes.x + (int)gx;
+      gs = S.c0 + ((int)gy * P.gridRes.z + (int)gz) * P.gridRes.x + (int)gx;
       pbuf<int>(FPnts, FGCELL)[i] = gs;
     } else {
-      gs = FParams.gridTotal;
+      gs = S.c0 + P.gridTotal;
       pbuf<int>(FPnts, FGCELL)[i] = FX_GRID_UNDEF;
     }
   }
@@ -143,13 +193,26 @@ extern "C" __global__ void insertParticles(int pnum) {
   if (active) pbuf<int>(FPnts, FGNDX)[i] = (int)slot;
 }
 
+extern "C" __global__ void insertParticles(int pnum) {
+  insert_impl(*cparams(&FParams), Span{0, pnum, 0, (int)blockIdx.x});
+}
+
+extern "C" __global__ void insertParticlesBatch(BatchArgs B) {
+  const FParams_t* P;
+  const Span S = batch_span(B, blockIdx.x, P);
+  insert_impl(*P, S);
+}
+
 // ------------------------------------------------------------- prefix scan
 
 // Block-local scan of 1024 uints, reference semantics (particles.cu:333-369):
 // block b covers [1024b, 1024b+1024); zeroff=1 gives the exclusive scan,
 // zeroff=0 the inclusive one; aux[b] (if given) receives the block total.
 // Each thread owns two adjacent elements; waves scan with DPP, the eight wave
-// totals are combined through LDS.
+// totals are combined through LDS.  A batch of scenes is scanned in one pass
+// over the concatenated cell tables: every scene's counts (tail bucket
+// included) add up to its particle count, so the running sum enters each
+// scene at its first particle.
 extern "C" __global__ __launch_bounds__(SCAN_BLOCKSIZE) void prefixSum(
     uint* input, uint* output, uint* aux, int len, int zeroff) {
   __shared__ uint wsum[SCAN_BLOCKSIZE / 64];
@@ -197,12 +260,12 @@ extern "C" __global__ __launch_bounds__(SCAN_BLOCKSIZE) void prefixFixup(
 // Index sort: AGRID[off[cell] + gndx] = i, the member list of every cell in
 // insertion (atomic) order.  Same role as fluids3.2's countingSortIndex
 // (fluids3.2/fluids/fluid_system_kern.cu:139-150); reads the pre-sort set
-// FPntTmp.  Out-of-range particles fill the tail bucket.
-extern "C" __global__ void countingSortIndex(int pnum) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= pnum) return;
+// FPntTmp.  Out-of-range particles fill their scene's tail bucket.
+__device__ __forceinline__ void sort_index_impl(const FParams_t& P, const Span S) {
+  const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
+  if (i >= S.p1) return;
   const int c = pbuf<const int>(FPntTmp, FGCELL)[i];
-  const int cell = (c == FX_GRID_UNDEF) ? FParams.gridTotal : c;
+  const int cell = (c == FX_GRID_UNDEF) ? S.c0 + P.gridTotal : c;
   const uint* off = pbuf<const uint>(FAccel, AGRIDOFF);
   pbuf<uint>(FAccel, AGRID)[off[cell] + pbuf<const int>(FPntTmp, FGNDX)[i]] = (uint)i;
 }
@@ -214,11 +277,11 @@ extern "C" __global__ void countingSortIndex(int pnum) {
 // Cells above FX_RANK_CAP members keep the atomic order.  The sort moves the
 // state that outlives a step (pos, vel, veval, clr, id) and the fresh gcell
 // and gndx; press and force are recomputed by the next two stages.
-extern "C" __global__ void countingSortFull(int pnum) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= pnum) return;
+__device__ __forceinline__ void sort_full_impl(const FParams_t& P, const Span S) {
+  const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
+  if (i >= S.p1) return;
   const int c = pbuf<const int>(FPntTmp, FGCELL)[i];
-  const int cell = (c == FX_GRID_UNDEF) ? FParams.gridTotal : c;
+  const int cell = (c == FX_GRID_UNDEF) ? S.c0 + P.gridTotal : c;
   const uint* off = pbuf<const uint>(FAccel, AGRIDOFF);
   const uint* cnt = pbuf<const uint>(FAccel, AGRIDCNT);
   const uint* grid = pbuf<const uint>(FAccel, AGRID);
@@ -241,49 +304,53 @@ extern "C" __global__ void countingSortFull(int pnum) {
   pbuf<int>(FPnts, FGNDX)[s] = rank;
 }
 
+extern "C" __global__ void countingSortIndex(int pnum) {
+  sort_index_impl(*cparams(&FParams), Span{0, pnum, 0, (int)blockIdx.x});
+}
+
+extern "C" __global__ void countingSortFull(int pnum) {
+  sort_full_impl(*cparams(&FParams), Span{0, pnum, 0, (int)blockIdx.x});
+}
+
+extern "C" __global__ void countingSortIndexBatch(BatchArgs B) {
+  const FParams_t* P;
+  const Span S = batch_span(B, blockIdx.x, P);
+  sort_index_impl(*P, S);
+}
+
+extern "C" __global__ void countingSortFullBatch(BatchArgs B) {
+  const FParams_t* P;
+  const Span S = batch_span(B, blockIdx.x, P);
+  sort_full_impl(*P, S);
+}
+
 // ----------------------------------------------------------------- density
 
 // Row decomposition of the stencil: gridAdj[(y*srch + z)*srch + x] =
 // (y*res.z + z)*res.x + x (particles.cpp:614-618) is contiguous in x, so the
 // srch cells of one (y, z) row are one contiguous run of sorted particles,
 // visited in the same order as the reference's per-cell loop.
-__device__ __forceinline__ void row_range(int base, int r, int srch, const uint* __restrict__ off,
+__device__ __forceinline__ void row_range(const FParams_t& P, int base, int r, const uint* __restrict__ off,
                                           const uint* __restrict__ cnt, int& jb, int& je) {
-  const int c0 = base + FParams.gridAdj[r * srch];
+  const int srch = P.gridSrch;
+  const int c0 = base + P.gridAdj[r * srch];
   const int c1 = c0 + srch - 1;
   jb = (int)off[c0];
   je = (int)(off[c1] + cnt[c1]);
 }
 
-// Wave-wide max / min with DPP row shifts and row broadcasts; lane 63 ends
-// up with the reduction of all 64 lanes and is read back as a scalar.
-template <bool kMax>
-__device__ __forceinline__ int wave_reduce(int v) {
-  const int id = kMax ? INT_MIN : INT_MAX;
-  int t;
-#define FX_RED_STEP(ctrl, rmask)                                         \
-  t = __builtin_amdgcn_update_dpp(id, v, ctrl, rmask, 0xf, false);      \
-  v = kMax ? max(v, t) : min(v, t);
-  FX_RED_STEP(0x111, 0xf)  // row_shr:1
-  FX_RED_STEP(0x112, 0xf)  // row_shr:2
-  FX_RED_STEP(0x114, 0xf)  // row_shr:4
-  FX_RED_STEP(0x118, 0xf)  // row_shr:8
-  FX_RED_STEP(0x142, 0xa)  // row_bcast:15
-  FX_RED_STEP(0x143, 0xc)  // row_bcast:31
-#undef FX_RED_STEP
-  return __builtin_amdgcn_readlane(v, 63);
+__device__ __forceinline__ int stencil_base(const FParams_t& P, int gc) {
+  return gc - (P.gridRes.z + 1) * P.gridRes.x - 1;
 }
-__device__ __forceinline__ int wave_max(int v) { return wave_reduce<true>(v); }
-__device__ __forceinline__ int wave_min(int v) { return wave_reduce<false>(v); }
 
 // Tiling.  A block owns the consecutive sorted particles [h0, h1) (in-range
-// particles only; out-of-range ones sit in the tail).  Their cells form one
-// contiguous index range [cf, cl], so for every stencil row the candidates of
-// all the block's particles form one contiguous run of sorted particles,
-// from the row start of cf to the row end of cl.  The block stages that run
-// in LDS in chunks of TILE_P with coalesced loads; each lane then reads the
-// part of the chunk that falls in its own row window.  Chunks are visited in
-// increasing order, so every lane still sees its candidates in the
+// particles only; out-of-range ones sit in the scene's tail).  Their cells
+// form one contiguous index range [cf, cl], so for every stencil row the
+// candidates of all the block's particles form one contiguous run of sorted
+// particles, from the row start of cf to the row end of cl.  The block stages
+// that run in LDS in chunks of TILE_P with coalesced loads; each lane then
+// reads the part of the chunk that falls in its own row window.  Chunks are
+// visited in increasing order, so every lane still sees its candidates in the
 // reference's order.
 #define TILE_P 1024  // float4 (pos, 0) per candidate, 16 KB
 
@@ -292,11 +359,12 @@ struct BlockSpan {
   int cf, cl;  // first and last cell of the block
 };
 
-__device__ __forceinline__ bool block_span(int h0, const int* __restrict__ gcell,
-                                           const uint* __restrict__ off, BlockSpan& s) {
-  const int nvalid = (int)off[FParams.gridTotal];  // start of the out-of-range tail
-  if (h0 >= nvalid) return false;
-  s.h1 = min(h0 + (int)blockDim.x, nvalid);
+__device__ __forceinline__ bool block_span(const FParams_t& P, const Span S, int h0,
+                                           const int* __restrict__ gcell, const uint* __restrict__ off,
+                                           BlockSpan& s) {
+  const int nvalid = (int)off[S.c0 + P.gridTotal];  // start of the scene's out-of-range tail
+  if (h0 >= nvalid || h0 >= S.p1) return false;
+  s.h1 = min(min(h0 + (int)blockDim.x, nvalid), S.p1);
   s.cf = gcell[h0];
   s.cl = gcell[s.h1 - 1];
   return true;
@@ -304,14 +372,13 @@ __device__ __forceinline__ bool block_span(int h0, const int* __restrict__ gcell
 
 // Number of bitmask words a particle of cell gc uses: one word-aligned run of
 // ceil(len/32) words per stencil row.
-__device__ __forceinline__ int nbr_words(int gc, const uint* __restrict__ off,
+__device__ __forceinline__ int nbr_words(const FParams_t& P, int gc, const uint* __restrict__ off,
                                          const uint* __restrict__ cnt) {
-  const int base = gc - (FParams.gridRes.z + 1) * FParams.gridRes.x - 1;
-  const int srch = FParams.gridSrch;
+  const int base = stencil_base(P, gc);
   int w = 0;
-  for (int r = 0; r < srch * srch; r++) {
+  for (int r = 0; r < P.gridSrch * P.gridSrch; r++) {
     int jb, je;
-    row_range(base, r, srch, off, cnt, jb, je);
+    row_range(P, base, r, off, cnt, jb, je);
     w += (je - jb + 31) >> 5;
   }
   return w;
@@ -324,12 +391,12 @@ __device__ __forceinline__ int nbr_words(int gc, const uint* __restrict__ off,
 // When the host provides the ANBRBITS and AFREC buffers the kernel also
 // records, for the force stage, one bit per stencil candidate (set for the
 // in-radius pairs; row r of particle i occupies words W_r .. W_r+ceil(len/32)-1
-// at [w * N + i]) and the particle's (pos, rho), (veval, 0) record.  A
-// particle whose candidates need more than FX_NBR_WORDS words records no bits.
-extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
-  __shared__ float4 tile[TILE_P];
+// at [w * N + i], N the size of the whole particle buffer) and the particle's
+// (pos, rho), (veval, 0) record.  A particle whose candidates need more than
+// FX_NBR_WORDS words records no bits.
+__device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, int N, float4* tile) {
   const int t = threadIdx.x;
-  const int h0 = blockIdx.x * blockDim.x;
+  const int h0 = S.p0 + S.blk * (int)blockDim.x;
   const int i = h0 + t;
   const int* __restrict__ gcell = pbuf<const int>(FPnts, FGCELL);
   const fx_float3* __restrict__ pos = pbuf<const fx_float3>(FPnts, FPOS);
@@ -337,23 +404,23 @@ extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
   const uint* __restrict__ cnt = pbuf<const uint>(FAccel, AGRIDCNT);
   uint* __restrict__ bits = pbuf<uint>(FAccel, ANBRBITS);
   const bool wbits = bits != nullptr && FAccel.mbuf[AFREC] != 0;
-  BlockSpan S;
-  if (h0 >= pnum || !block_span(h0, gcell, off, S)) return;  // uniform per block
-  const bool act = i < S.h1;
-  const int gc = act ? gcell[i] : S.cf;
+  BlockSpan B;
+  if (!block_span(P, S, h0, gcell, off, B)) return;  // uniform per block
+  const bool act = i < B.h1;
+  const int gc = act ? gcell[i] : B.cf;
   fx_float3 p = fx_float3{0.0f, 0.0f, 0.0f};
   if (act) p = pos[i];
-  const int shift = (FParams.gridRes.z + 1) * FParams.gridRes.x + 1;
-  const int srch = FParams.gridSrch;
-  const float rd2 = FParams.rd2, d2 = FParams.d2;
-  const size_t N = (size_t)pnum;
-  const bool rec = wbits && act && nbr_words(gc, off, cnt) <= FX_NBR_WORDS;
+  const int shift = (P.gridRes.z + 1) * P.gridRes.x + 1;
+  const int srch = P.gridSrch;
+  const float rd2 = P.rd2, d2 = P.d2;
+  const size_t NN = (size_t)N;
+  const bool rec = wbits && act && nbr_words(P, gc, off, cnt) <= FX_NBR_WORDS;
   float sum = 0.0f;
   int W = 0;  // first word of the current row
   for (int r = 0; r < srch * srch; r++) {
-    const int adj = FParams.gridAdj[r * srch] - shift;
-    const int lo = (int)off[S.cf + adj];
-    const int hi = (int)(off[S.cl + adj + srch - 1] + cnt[S.cl + adj + srch - 1]);
+    const int adj = P.gridAdj[r * srch] - shift;
+    const int lo = (int)off[B.cf + adj];
+    const int hi = (int)(off[B.cl + adj + srch - 1] + cnt[B.cl + adj + srch - 1]);
     int jb = 0, je = 0;
     if (act) {
       jb = (int)off[gc + adj];
@@ -381,7 +448,7 @@ extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
         if (rec) {
           acc |= pair ? bit : 0u;
           if (bit == 0x80000000u) {  // word complete
-            bits[(size_t)wcur * N + i] = acc;
+            bits[(size_t)wcur * NN + i] = acc;
             acc = 0;
             wcur++;
           }
@@ -390,11 +457,11 @@ extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
       }
       __syncthreads();
     }
-    if (rec && bit != 1u) bits[(size_t)wcur * N + i] = acc;  // the row's partial last word
+    if (rec && bit != 1u) bits[(size_t)wcur * NN + i] = acc;  // the row's partial last word
     W += (je - jb + 31) >> 5;
   }
   if (act) {
-    sum = sum * FParams.pmass * FParams.poly6kern;
+    sum = sum * P.pmass * P.poly6kern;
     if (sum == 0.0f) sum = 1.0f;
     pbuf<float>(FPnts, FPRESS)[i] = sum;
     if (wbits) {
@@ -406,21 +473,32 @@ extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
   }
 }
 
+extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
+  __shared__ float4 tile[TILE_P];
+  pressure_impl(*cparams(&FParams), Span{0, pnum, 0, (int)blockIdx.x}, pnum, tile);
+}
+
+extern "C" __global__ __launch_bounds__(1024) void computePressureBatch(BatchArgs B, int ntotal) {
+  __shared__ float4 tile[TILE_P];
+  const FParams_t* P;
+  const Span S = batch_span(B, blockIdx.x, P);
+  pressure_impl(*P, S, ntotal, tile);
+}
+
 // ------------------------------------------------------------------- force
 
 // Force (particles.cu:188-225): for 0 < d2 < rd2, r = sqrt(d2 * d2s),
 // c = h - r, p = (rho - rho0) * k,
 // pterm = (ss * -0.5) * c * spiky * (pi + pj) / r,
-// F += (pterm * dist + vterm * (veval_j - veval_i)) * c * (1 / (rho_i rho_j)).
+// F += (pterm * dist + vterm * (veval_j - veval_i)) * c / (rho_i rho_j).
 //
 // With the density stage's bitmask the kernel never re-tests candidates.
 // Each lane turns the set bits of its words into neighbour indices in a
 // per-lane ring in LDS; the wave then evaluates queued pairs in batches: a
 // batch takes from every lane as many pairs as the shortest active queue
 // holds (more only when a queue must make room), so the expensive pair term
-// (square root, two IEEE divisions) runs with nearly every lane busy.  Each
-// lane adds its own pairs in candidate order.  Waves are independent: no
-// block-wide barrier.
+// runs with nearly every lane busy.  Each lane adds its own pairs in
+// candidate order.  Waves are independent: no block-wide barrier.
 #define FQ_SLOTS 1024  // ring slots per block (16 per lane at 64 threads)
 
 struct ForceI {
@@ -435,41 +513,40 @@ struct ForceI {
 //   A = ((ss * -0.5) * spiky) c^2 (p_i + p_j) inv,  B = vterm c r inv,
 // which is the reference's (pterm dist + vterm dv) c / D with
 // pterm = ss (-0.5) c spiky (p_i + p_j) / r.
-__device__ __forceinline__ void force_pair(ForceI& I, const float4 a, const float4 v) {
+__device__ __forceinline__ void force_pair(const FParams_t& P, ForceI& I, const float4 a, const float4 v) {
   const float dx = I.px - a.x, dy = I.py - a.y, dz = I.pz - a.z;
   const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-  const float rr = sqrtf(dsq * FParams.d2);
-  const float c = FParams.psmoothradius - rr;
+  const float rr = sqrtf(dsq * P.d2);
+  const float c = P.psmoothradius - rr;
   const float dj = a.w;
-  const float pj = (dj - FParams.prest_dens) * FParams.pintstiff;
+  const float pj = (dj - P.prest_dens) * P.pintstiff;
   const float inv = 1.0f / (rr * (I.di * dj));
-  const float kp = (FParams.sim_scale * -0.5f) * FParams.spikykern;
+  const float kp = (P.sim_scale * -0.5f) * P.spikykern;
   const float A = (kp * (c * c)) * (I.pi + pj) * inv;
-  const float B = (FParams.vterm * c) * rr * inv;
-  I.fx = fmaf(A, dx, fmaf(B, v.x - I.vx, I.fx));
-  I.fy = fmaf(A, dy, fmaf(B, v.y - I.vy, I.fy));
-  I.fz = fmaf(A, dz, fmaf(B, v.z - I.vz, I.fz));
+  const float Bv = (P.vterm * c) * rr * inv;
+  I.fx = fmaf(A, dx, fmaf(Bv, v.x - I.vx, I.fx));
+  I.fy = fmaf(A, dy, fmaf(Bv, v.y - I.vy, I.fy));
+  I.fz = fmaf(A, dz, fmaf(Bv, v.z - I.vz, I.fz));
 }
 
-__device__ __forceinline__ bool is_pair(const ForceI& I, const float4 a) {
+__device__ __forceinline__ bool is_pair(const FParams_t& P, const ForceI& I, const float4 a) {
   const float dx = I.px - a.x, dy = I.py - a.y, dz = I.pz - a.z;
   const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-  return dsq < FParams.rd2 && dsq > 0.0f;
+  return dsq < P.rd2 && dsq > 0.0f;
 }
 
-extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
-  __shared__ uint fq[FQ_SLOTS];
+__device__ __forceinline__ void force_impl(const FParams_t& P, const Span S, int N, uint* fq) {
   const int t = threadIdx.x;
-  const int i = xcd_block() * blockDim.x + t;
+  const int i = S.p0 + S.blk * (int)blockDim.x + t;
   const uint* __restrict__ off = pbuf<const uint>(FAccel, AGRIDOFF);
   const uint* __restrict__ cnt = pbuf<const uint>(FAccel, AGRIDCNT);
   const uint* __restrict__ bits = pbuf<const uint>(FAccel, ANBRBITS);
   const float4* __restrict__ fr = pbuf<const float4>(FAccel, AFREC);
-  const int srch = FParams.gridSrch;
+  const int srch = P.gridSrch;
   int gc = FX_GRID_UNDEF;
-  if (i < pnum) gc = pbuf<const int>(FPnts, FGCELL)[i];
+  if (i < S.p1) gc = pbuf<const int>(FPnts, FGCELL)[i];
   const bool act = gc != FX_GRID_UNDEF;
-  const int base = gc - (FParams.gridRes.z + 1) * FParams.gridRes.x - 1;
+  const int base = stencil_base(P, gc);
   ForceI I;
   I.px = I.py = I.pz = I.vx = I.vy = I.vz = 0.0f;
   I.di = 1.0f;
@@ -487,14 +564,14 @@ extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
       I.px = p.x; I.py = p.y; I.pz = p.z;
       I.vx = v.x; I.vy = v.y; I.vz = v.z;
       I.di = rho[i];
-      I.pi = (I.di - FParams.prest_dens) * FParams.pintstiff;
+      I.pi = (I.di - P.prest_dens) * P.pintstiff;
       for (int r = 0; r < srch * srch; r++) {
         int jb, je;
-        row_range(base, r, srch, off, cnt, jb, je);
+        row_range(P, base, r, off, cnt, jb, je);
         for (int j = jb; j < je; j++) {
           const fx_float3 q = pos[j], u = vev[j];
           const float4 a = make_float4(q.x, q.y, q.z, rho[j]);
-          if (is_pair(I, a)) force_pair(I, a, make_float4(u.x, u.y, u.z, 0.0f));
+          if (is_pair(P, I, a)) force_pair(P, I, a, make_float4(u.x, u.y, u.z, 0.0f));
         }
       }
       pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{I.fx, I.fy, I.fz};
@@ -507,21 +584,22 @@ extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
     I.px = a.x; I.py = a.y; I.pz = a.z;
     I.vx = v.x; I.vy = v.y; I.vz = v.z;
     I.di = a.w;
-    I.pi = (I.di - FParams.prest_dens) * FParams.pintstiff;
+    I.pi = (I.di - P.prest_dens) * P.pintstiff;
   }
   // a particle without recorded bits (too many candidates) tests its own
-  const bool ovf = act && nbr_words(gc, off, cnt) > FX_NBR_WORDS;
+  const bool ovf = act && nbr_words(P, gc, off, cnt) > FX_NBR_WORDS;
   if (ovf) {
     for (int r = 0; r < srch * srch; r++) {
       int jb, je;
-      row_range(base, r, srch, off, cnt, jb, je);
+      row_range(P, base, r, off, cnt, jb, je);
       for (int j = jb; j < je; j++) {
         const float4 a = fr[2 * (size_t)j];
-        if (is_pair(I, a)) force_pair(I, a, fr[2 * (size_t)j + 1]);
+        if (is_pair(P, I, a)) force_pair(P, I, a, fr[2 * (size_t)j + 1]);
       }
     }
   }
   const bool use = act && !ovf;
+  const size_t NN = (size_t)N;
   const int fqn = FQ_SLOTS / (int)blockDim.x;  // ring entries per lane
   if (fqn < 16) {
     // blocks too large for a ring that holds a half-word per lane: evaluate
@@ -529,14 +607,14 @@ extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
     int W = 0;
     for (int r = 0; r < srch * srch; r++) {
       int jb = 0, je = 0;
-      if (use) row_range(base, r, srch, off, cnt, jb, je);
+      if (use) row_range(P, base, r, off, cnt, jb, je);
       const int nw = (je - jb + 31) >> 5;
       for (int w = 0; w < nw; w++) {
-        uint m = bits[(size_t)(W + w) * pnum + i];
+        uint m = bits[(size_t)(W + w) * NN + i];
         while (m != 0u) {
           const uint j = (uint)(jb + (w << 5) + __builtin_ctz(m));
           m &= m - 1u;
-          force_pair(I, fr[2 * (size_t)j], fr[2 * (size_t)j + 1]);
+          force_pair(P, I, fr[2 * (size_t)j], fr[2 * (size_t)j + 1]);
         }
       }
       W += nw;
@@ -544,7 +622,6 @@ extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
     if (act) pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{I.fx, I.fy, I.fz};
     return;
   }
-  const size_t N = (size_t)pnum;
   int head = 0, qn = 0;  // ring holds entries head .. head+qn-1 (mod fqn)
   int W = 0;
   // evaluates the ring's first f entries of every lane (fewer if it holds
@@ -558,7 +635,7 @@ extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
       const uint j = ok ? fq[head * (int)blockDim.x + t] : 0u;
       const float4 a = fr[2 * (size_t)j], v = fr[2 * (size_t)j + 1];
       ForceI J = I;
-      force_pair(J, a, v);
+      force_pair(P, J, a, v);
       if (ok) {
         I.fx = J.fx;
         I.fy = J.fy;
@@ -570,12 +647,12 @@ extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
   };
   for (int r = 0; r < srch * srch; r++) {
     int jb = 0, je = 0;
-    if (use) row_range(base, r, srch, off, cnt, jb, je);
+    if (use) row_range(P, base, r, off, cnt, jb, je);
     const int nw = (je - jb + 31) >> 5;
     const int maxw = min(wave_max(nw), FX_NBR_WORDS);
     for (int w = 0; w < maxw; w++) {
       uint word = 0;
-      if (w < nw) word = bits[(size_t)(W + w) * N + i];
+      if (w < nw) word = bits[(size_t)(W + w) * NN + i];
       for (int h = 0; h < 32; h += 16) {  // a word in two halves of 16 candidates
         uint m = (word >> h) & 0xffffu;
         const int pc = __popc(m);
@@ -601,6 +678,18 @@ extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
   if (act) pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{I.fx, I.fy, I.fz};
 }
 
+extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
+  __shared__ uint fq[FQ_SLOTS];
+  force_impl(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, fq);
+}
+
+extern "C" __global__ __launch_bounds__(1024) void computeForceBatch(BatchArgs B, int ntotal) {
+  __shared__ uint fq[FQ_SLOTS];
+  const FParams_t* P;
+  const Span S = batch_span(B, xcd_block(), P);
+  force_impl(*P, S, ntotal, fq);
+}
+
 // ------------------------------------------------------ neighbour statistics
 
 // Sums over all in-range particles of the stencil candidates (members of the
@@ -608,27 +697,26 @@ extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
 // on the cell-sorted set: out[0] += C, out[1] += P.  The measurement behind
 // the flop model of the density and force stages; the role of the
 // reference's neighbour-count validation (fluids4.0/fluid_system.cpp:959-1011).
-extern "C" __global__ void fxCountPairs(unsigned long long* out, int pnum) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void count_pairs_impl(const FParams_t& P, const Span S, unsigned long long* out) {
+  const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
   unsigned long long nc = 0, np = 0;
-  if (i < pnum) {
+  if (i < S.p1) {
     const int gc = pbuf<const int>(FPnts, FGCELL)[i];
     if (gc != FX_GRID_UNDEF) {
       const fx_float3* __restrict__ pos = pbuf<const fx_float3>(FPnts, FPOS);
       const uint* __restrict__ off = pbuf<const uint>(FAccel, AGRIDOFF);
       const uint* __restrict__ cnt = pbuf<const uint>(FAccel, AGRIDCNT);
       const fx_float3 p = pos[i];
-      const int base = gc - (FParams.gridRes.z + 1) * FParams.gridRes.x - 1;
-      const int srch = FParams.gridSrch;
-      for (int r = 0; r < srch * srch; r++) {
+      const int base = stencil_base(P, gc);
+      for (int r = 0; r < P.gridSrch * P.gridSrch; r++) {
         int jb, je;
-        row_range(base, r, srch, off, cnt, jb, je);
+        row_range(P, base, r, off, cnt, jb, je);
         nc += (unsigned long long)(je - jb);
         for (int j = jb; j < je; j++) {
           const fx_float3 q = pos[j];
           const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
           const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-          np += (dsq < FParams.rd2 && dsq > 0.0f) ? 1u : 0u;
+          np += (dsq < P.rd2 && dsq > 0.0f) ? 1u : 0u;
         }
       }
     }
@@ -644,6 +732,16 @@ extern "C" __global__ void fxCountPairs(unsigned long long* out, int pnum) {
   }
 }
 
+extern "C" __global__ void fxCountPairs(unsigned long long* out, int pnum) {
+  count_pairs_impl(*cparams(&FParams), Span{0, pnum, 0, (int)blockIdx.x}, out);
+}
+
+extern "C" __global__ void fxCountPairsBatch(BatchArgs B, unsigned long long* out) {
+  const FParams_t* P;
+  const Span S = batch_span(B, blockIdx.x, P);
+  count_pairs_impl(*P, S, out);
+}
+
 // ----------------------------------------------------------------- advance
 
 // Integrator (particles.cu:237-322): penalty walls on bound_min/bound_max
@@ -652,7 +750,7 @@ extern "C" __global__ void fxCountPairs(unsigned long long* out, int pnum) {
 // acceleration limit, the velocity limit applied to the old velocity, and the
 // leapfrog update.  GRID_UNDEF particles are parked at (-1000)^3 with zero
 // velocity.
-__device__ __forceinline__ void advance_one(int i, float time, float dt, float ss) {
+__device__ __forceinline__ void advance_one(const FParams_t& P, int i, float time, float dt, float ss) {
   fx_float3* __restrict__ posb = pbuf<fx_float3>(FPnts, FPOS);
   fx_float3* __restrict__ velb = pbuf<fx_float3>(FPnts, FVEL);
   if (pbuf<const int>(FPnts, FGCELL)[i] == FX_GRID_UNDEF) {
@@ -665,11 +763,11 @@ __device__ __forceinline__ void advance_one(int i, float time, float dt, float s
   fx_float3 vel = velb[i];
   const fx_float3 ve = vevb[i];
   const fx_float3 f = pbuf<const fx_float3>(FPnts, FFORCE)[i];
-  const float m = FParams.pmass;
+  const float m = P.pmass;
   float ax = f.x * m, ay = f.y * m, az = f.z * m;
-  const float rad = FParams.pradius, stiff = FParams.bound_stiff, damp = FParams.bound_damp;
-  const float slope = FParams.bound_slope, fric = FParams.bound_friction;
-  const fx_float3 bmin = FParams.bound_min, bmax = FParams.bound_max;
+  const float rad = P.pradius, stiff = P.bound_stiff, damp = P.bound_damp;
+  const float slope = P.bound_slope, fric = P.bound_friction;
+  const fx_float3 bmin = P.bound_min, bmax = P.bound_max;
   float diff, adj, nx, ny, nz;
 
   // Y-min, sloped floor with friction
@@ -695,8 +793,8 @@ __device__ __forceinline__ void advance_one(int i, float time, float dt, float s
     az += nz - ve.z * fric;
   }
   // X-min, moving wall; its stiffness term is scaled by 'wall'
-  const float sarg = time * FParams.bound_wall_freq + pos.z / 400.0f;
-  const float wall = (float)(((double)sinf(sarg) * 0.5 + 0.5) * (double)FParams.bound_wall_force);
+  const float sarg = time * P.bound_wall_freq + pos.z / 400.0f;
+  const float wall = (float)(((double)sinf(sarg) * 0.5 + 0.5) * (double)P.bound_wall_force);
   diff = rad - (pos.x - (bmin.x + wall)) * ss;
   if (diff > FX_EPSILON) {
     nx = 1.0f; ny = 0.0f; nz = 0.0f;
@@ -725,19 +823,19 @@ __device__ __forceinline__ void advance_one(int i, float time, float dt, float s
     ax += nx * adj; ay += ny * adj; az += nz * adj;
   }
   // gravity
-  ax += FParams.gravity.x;
-  ay += FParams.gravity.y;
-  az += FParams.gravity.z;
+  ax += P.gravity.x;
+  ay += P.gravity.y;
+  az += P.gravity.z;
   // acceleration limit
   float speed = ax * ax + ay * ay + az * az;
-  if (speed > FParams.AL2) {
-    const float s = FParams.AL / sqrtf(speed);
+  if (speed > P.AL2) {
+    const float s = P.AL / sqrtf(speed);
     ax *= s; ay *= s; az *= s;
   }
   // velocity limit, applied to the old velocity
   speed = vel.x * vel.x + vel.y * vel.y + vel.z * vel.z;
-  if (speed > FParams.VL2) {
-    const float s = FParams.VL / sqrtf(speed);
+  if (speed > P.VL2) {
+    const float s = P.VL / sqrtf(speed);
     vel.x *= s; vel.y *= s; vel.z *= s;
   }
   // leapfrog
@@ -751,20 +849,29 @@ __device__ __forceinline__ void advance_one(int i, float time, float dt, float s
 extern "C" __global__ void advanceParticles(float time, float dt, float ss, int numPnts) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= numPnts) return;
-  advance_one(i, time, dt, ss);
+  advance_one(*cparams(&FParams), i, time, dt, ss);
 }
 
 // Same integrator with the simulation time read from device memory, for
 // captured step graphs whose launches cannot carry a per-step time argument.
 // fxClockTick advances that clock by dt (the host's AdvanceTime,
 // particles.cpp:396-400).
-extern "C" __global__ void advanceParticlesClock(const float* clock, float dt, float ss,
-                                                 int numPnts) {
+extern "C" __global__ void advanceParticlesClock(const float* clock, float dt, float ss, int numPnts) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= numPnts) return;
-  advance_one(i, clock[0], dt, ss);
+  advance_one(*cparams(&FParams), i, clock[0], dt, ss);
 }
 
 extern "C" __global__ void fxClockTick(float* clock, float dt) {
   if (blockIdx.x == 0 && threadIdx.x == 0) clock[0] = clock[0] + dt;
+}
+
+// Batch integrator: the time comes from clock[0] when clock is given, else
+// from the argument; dt and the scale from each scene's parameters.
+extern "C" __global__ void advanceParticlesBatch(BatchArgs B, const float* clock, float time) {
+  const FParams_t* P;
+  const Span S = batch_span(B, blockIdx.x, P);
+  const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
+  if (i >= S.p1) return;
+  advance_one(*P, i, clock != nullptr ? clock[0] : time, P->dt, P->sim_scale);
 }

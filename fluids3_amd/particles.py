@@ -28,10 +28,15 @@ SIGNATURES = {
     "fx_create": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "fx_destroy": (None, [_P]),
     "fx_restart": (C.c_int, [_P, C.POINTER(abi.Scene)]),
+    "fx_restart_batch": (C.c_int, [_P, C.POINTER(abi.Scene), C.c_int32]),
+    "fx_num_scenes": (C.c_int32, [_P]),
+    "fx_scene_range": (C.c_int, [_P, C.c_int32] + [C.POINTER(C.c_int32)] * 4),
+    "fx_scene_params": (C.c_int, [_P, C.c_int32, C.POINTER(abi.FParams)]),
     "fx_get_params": (C.c_int, [_P, C.POINTER(abi.FParams)]),
     "fx_set_params": (C.c_int, [_P, C.POINTER(abi.FParams)]),
     "fx_num_points": (C.c_int32, [_P]),
     "fx_grid_total": (C.c_int32, [_P]),
+    "fx_cell_table_size": (C.c_int32, [_P]),
     "fx_buffer_stride": (C.c_int32, [C.c_int32]),
     "fx_upload": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "fx_download": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
@@ -131,6 +136,25 @@ class Particles:
     def Restart(self, scene):
         check(lib().fx_restart(self._h, C.byref(scene)), "fx_restart")
 
+    def RestartBatch(self, scene_list):
+        """Many independent scenes in one system, each stage one launch over all."""
+        arr = (abi.Scene * len(scene_list))(*scene_list)
+        check(lib().fx_restart_batch(self._h, arr, len(scene_list)), "fx_restart_batch")
+
+    def NumScenes(self):
+        return int(lib().fx_num_scenes(self._h))
+
+    def SceneRange(self, s):
+        """(first particle, particles, first cell, cells incl. the tail slot) of scene s."""
+        v = [C.c_int32() for _ in range(4)]
+        check(lib().fx_scene_range(self._h, s, *[C.byref(x) for x in v]), "fx_scene_range")
+        return tuple(x.value for x in v)
+
+    def SceneParams(self, s):
+        p = abi.FParams()
+        check(lib().fx_scene_params(self._h, s, C.byref(p)), "fx_scene_params")
+        return p
+
     @property
     def params(self):
         p = abi.FParams()
@@ -145,6 +169,9 @@ class Particles:
 
     def GridTotal(self):
         return int(lib().fx_grid_total(self._h))
+
+    def CellTableSize(self):
+        return int(lib().fx_cell_table_size(self._h))
 
     def SetBlockSize(self, threads):
         check(lib().fx_set_block_size(self._h, threads), "fx_set_block_size")
@@ -221,13 +248,13 @@ class Particles:
 
     def state(self):
         """All particle buffers and the grid counts/offsets, as numpy arrays."""
-        G = self.GridTotal()
+        ncell = self.CellTableSize()
         st = {name: self.Retrieve(b) for name, b in (
             ("pos", abi.FPOS), ("vel", abi.FVEL), ("veval", abi.FVEVAL), ("force", abi.FFORCE),
             ("press", abi.FPRESS), ("clr", abi.FCLR), ("gcell", abi.FGCELL), ("gndx", abi.FGNDX),
             ("id", abi.FID))}
-        st["cnt"] = self.RetrieveAccel(abi.AGRIDCNT, G + 1)
-        st["off"] = self.RetrieveAccel(abi.AGRIDOFF, G + 1)
+        st["cnt"] = self.RetrieveAccel(abi.AGRIDCNT, ncell)
+        st["off"] = self.RetrieveAccel(abi.AGRIDOFF, ncell)
         st["grid"] = self.RetrieveAccel(abi.AGRID, self.NumPoints())
         st["time"] = self.params.time
         return st

@@ -94,6 +94,19 @@ void fx_destroy(fx_system* sys);
  * particle and grid buffers, fill the lattice, upload everything. */
 int fx_restart(fx_system* sys, const fx_scene* scene);
 
+/* A batch of independent scenes in one system (no reference counterpart:
+ * the reference simulates one scene per Particles object).  The scenes'
+ * particles are concatenated in the particle buffers (scene s occupies
+ * [first_particle, first_particle + num_particles)), their cell tables
+ * (each with its out-of-range slot) are concatenated in AGRIDCNT/AGRIDOFF,
+ * and every stage is one launch over all scenes.  Each scene keeps its own
+ * box, grid and parameters; all share the time step count. */
+int fx_restart_batch(fx_system* sys, const fx_scene* scenes, int32_t count);
+int32_t fx_num_scenes(const fx_system* sys);
+int fx_scene_range(const fx_system* sys, int32_t scene, int32_t* first_particle, int32_t* num_particles,
+                   int32_t* first_cell, int32_t* num_cells);
+int fx_scene_params(const fx_system* sys, int32_t scene, FParams_t* out);
+
 /* FParams_t currently in use (host copy). */
 int fx_get_params(const fx_system* sys, FParams_t* out);
 /* Particles::UpdateParams (particles.cpp:657-679): takes the input fields of
@@ -103,6 +116,8 @@ int fx_set_params(fx_system* sys, const FParams_t* p);
 
 int32_t fx_num_points(const fx_system* sys);
 int32_t fx_grid_total(const fx_system* sys);
+/* Entries of AGRIDCNT / AGRIDOFF: gridTotal + 1, or the sum over a batch. */
+int32_t fx_cell_table_size(const fx_system* sys);
 
 /* ---- buffers (DataX::Commit / Retrieve, datax.cpp:124-137) ---- */
 
