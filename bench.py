@@ -33,6 +33,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (matrix and packed vector)
 
+TRAFFIC_FILES = {"cfg1": "traffic_dambreak_4M_literal.json", "cfg1_rest": "traffic_dambreak_4M_rest.json",
+                 "cfg2": "traffic_batch64_dambreak.json"}
+
 BYTES_PER_PARTICLE = {"insert": 20, "prefix": 0, "sort": 92, "pressure": 20, "force": 44, "advance": 88}
 BYTES_PER_CELL = {"insert": 4, "prefix": 8, "sort": 4, "pressure": 0, "force": 0, "advance": 0}
 
@@ -65,6 +68,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every core this process may use")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-single", action="store_true", help="cfg2: skip the one-scene-at-a-time comparison")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (reference for the FETCH_SIZE / WRITE_SIZE "
                          "counters of a kernel of known traffic) and exit")
@@ -171,7 +175,34 @@ def main():
     roof["kernel"] = {"insert": "insertParticles", "prefix": "prefixSum", "sort": "countingSortFull",
                       "pressure": "computePressure", "force": "computeForce", "advance": "advanceParticles"}[dom]
     roof["kernel_ms"] = round(stage_ms[dom], 5)
+    # HBM bytes per launch of that kernel from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this workload (copy-calibrated)
     roof["traffic"] = None
+    tf = os.path.join(ROOT, "profiles", TRAFFIC_FILES.get(args.workload, "-"))
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            t = json.load(fh)
+        if roof["kernel"] in t.get("bytes_per_launch", {}):
+            roof["traffic"] = round(t["bytes_per_launch"][roof["kernel"]])
+            roof["traffic_source"] = "profiles/" + os.path.basename(tf)
+
+    # the batch's reference point: the same scenes stepped one at a time, each
+    # as its own single-scene system (same warm-up, same graph-timed region)
+    single = None
+    if batch and not args.no_single:
+        one = Particles()
+        tot_ms = 0.0
+        for scene in sc:
+            one.Restart(scene)
+            one.Run(W)
+            one.Synchronize()
+            ms, _ = one.BenchSteps(K, use_graph=use_graph)
+            tot_ms += ms
+        one.close()
+        v1 = N * K / (tot_ms * 1e-3)
+        single = {"value": round(v1, 1), "ms_per_step": round(tot_ms / K, 5),
+                  "batch_over_single": round(value / v1, 3),
+                  "how": f"the same {len(sc)} scenes run one after another, each as a single-scene system"}
 
     cpu = None
     if want_cpu:
@@ -202,7 +233,8 @@ def main():
         "data": "synthetic (seeded jittered lattice)",
         "config": {"workload": describe(args.workload, sc, N, G), "particles": N, "grid_cells": G,
                    "graph": use_graph, "candidates_per_particle": round(C, 2),
-                   "pairs_per_particle": round(P, 2), "stages": stages},
+                   "pairs_per_particle": round(P, 2), "stages": stages,
+                   **({"single_scene": single} if single else {})},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
