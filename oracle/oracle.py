@@ -47,6 +47,15 @@ def lib():
         L.orc_pairs_grid.argtypes = [P, C.c_int, P, P, P, P, P, P]
         L.orc_pairs_brute.restype = C.c_int
         L.orc_pairs_brute.argtypes = [P, C.c_int, P, P, P]
+        for name, real in (("f32", C.c_float), ("f64", C.c_double)):
+            fn = getattr(L, "orc_grad_" + name)
+            fn.restype = C.c_int
+            fn.argtypes = [P, C.c_int, P, P, P, P, P, real, C.c_int, C.c_int, C.c_double, C.c_double, P,
+                           C.c_int, P, P, P, P, P]
+            fn = getattr(L, "orc_loss_" + name)
+            fn.restype = C.c_double
+            fn.argtypes = [P, C.c_int, P, P, P, P, P, real, C.c_int, C.c_int, C.c_double, C.c_double, P,
+                           C.c_int]
         _lib = L
     return _lib
 
@@ -140,3 +149,55 @@ def pairs_brute(params, pos):
     dens = np.zeros(n, np.float64)
     lib().orc_pairs_brute(C.byref(params), n, _ptr(pos), _ptr(pairs), _ptr(dens))
     return pairs, dens
+
+
+def _initial(state, dt):
+    pos = np.ascontiguousarray(state["pos"], dt).reshape(-1, 3)
+    n = pos.shape[0]
+    vel = np.ascontiguousarray(state.get("vel", np.zeros((n, 3))), dt).reshape(-1, 3)
+    veval = np.ascontiguousarray(state.get("veval", vel), dt).reshape(-1, 3)
+    clr = np.ascontiguousarray(state.get("clr", np.zeros(n)), np.uint32)
+    ids = np.ascontiguousarray(state.get("id", np.arange(n)), np.uint32)
+    return n, pos, vel, veval, clr, ids
+
+
+def _coef(coef):
+    return (0, 0.0, 0.0) if coef is None else (1, float(coef[0]), float(coef[1]))
+
+
+def grad(params, state, nsteps, target, fp64=False, coef=None, threads=1):
+    """Loss (1/n) sum |x_T - x*|^2 of an nsteps rollout from `state` (pos, vel,
+    veval, clr, id, time) against target (n x 3, by particle id), and its
+    gradient with respect to the initial vel and veval (by id), viscosity and
+    internal stiffness.  coef = (visc, k) evaluates the force with those
+    values in the rollout's precision instead of the FParams_t floats."""
+    dt = np.float64 if fp64 else np.float32
+    n, pos, vel, veval, clr, ids = _initial(state, dt)
+    tgt = np.ascontiguousarray(target, dt).reshape(-1, 3)
+    assert tgt.shape[0] == n
+    gv = np.zeros((n, 3), dt)
+    ge = np.zeros((n, 3), dt)
+    loss, gvisc, gk = C.c_double(), C.c_double(), C.c_double()
+    use, visc, k = _coef(coef)
+    fn = lib().orc_grad_f64 if fp64 else lib().orc_grad_f32
+    fn(C.byref(params), n, _ptr(pos), _ptr(vel), _ptr(veval), _ptr(clr), _ptr(ids), float(state.get("time", 0.0)),
+       nsteps, use, visc, k, _ptr(tgt), threads, C.byref(loss), _ptr(gv), _ptr(ge), C.byref(gvisc), C.byref(gk))
+    return {"loss": loss.value, "gvel": gv, "gveval": ge, "gvisc": gvisc.value, "gk": gk.value}
+
+
+def loss(params, state, nsteps, target, fp64=True, coef=None, threads=1):
+    """Loss of an nsteps rollout (see grad), no gradient."""
+    dt = np.float64 if fp64 else np.float32
+    n, pos, vel, veval, clr, ids = _initial(state, dt)
+    tgt = np.ascontiguousarray(target, dt).reshape(-1, 3)
+    use, visc, k = _coef(coef)
+    fn = lib().orc_loss_f64 if fp64 else lib().orc_loss_f32
+    return fn(C.byref(params), n, _ptr(pos), _ptr(vel), _ptr(veval), _ptr(clr), _ptr(ids),
+              float(state.get("time", 0.0)), nsteps, use, visc, k, _ptr(tgt), threads)
+
+
+def by_id(values, ids):
+    """Reorders per-slot values into particle-id order."""
+    out = np.empty_like(values)
+    out[np.asarray(ids, np.int64)] = values
+    return out

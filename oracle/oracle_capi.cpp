@@ -17,11 +17,14 @@
 //   orc_run (advance)     integrator                pinned: free fall, each wall band,
 //                                                     both limiters, GRID_UNDEF park
 //   orc_pairs_*           neighbour counts          pinned: brute force
+//   orc_grad_*            adjoint of a rollout      pinned (FP64): central differences of
+//                                                     orc_loss_f64 in every input kind,
+//                                                     a two-particle closed form
 #include <chrono>
 #include <cmath>
 #include <cstring>
 
-#include "sph_oracle.hpp"
+#include "sph_adjoint.hpp"
 
 namespace {
 
@@ -203,5 +206,70 @@ double orc_time_f32(const FParams_t* P, int n, float* pos, float* vel, float* ve
   store(S, pos, vel, veval, force, press, clr, id, gcell, gndx, cnt, off, grid, time);
   return std::chrono::duration<double>(t1 - t0).count();
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- adjoint
+namespace {
+
+template <class R>
+void load_initial(oracle::State<R>& S, const FParams_t* P, int n, const R* pos, const R* vel, const R* veval,
+                  const uint32_t* clr, const uint32_t* id, R time, int use_coef, double visc, double k) {
+  S.resize(n, P->gridTotal);
+  for (int i = 0; i < n; i++) {
+    S.pos[i] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    S.vel[i] = {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
+    S.veval[i] = {veval[3 * i], veval[3 * i + 1], veval[3 * i + 2]};
+    S.clr[i] = clr[i];
+    S.id[i] = id[i];
+  }
+  S.time = time;
+  if (use_coef) {
+    S.coef_set = true;
+    S.vterm = (R)P->lapkern * (R)visc;
+    S.stiff = (R)k;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Loss of an nsteps rollout and its adjoint: gradients of the loss with
+// respect to the initial vel and veval of every particle (by id, 3n each),
+// viscosity (through vterm = lapkern visc) and internal stiffness.  With
+// use_coef the force uses visc and k in the state's precision, otherwise
+// the FParams_t values.
+#define ORC_GRAD(NAME, R)                                                                                 \
+  int NAME(const FParams_t* P, int n, const R* pos, const R* vel, const R* veval, const uint32_t* clr,  \
+           const uint32_t* id, R time, int nsteps, int use_coef, double visc, double k, const R* target,   \
+           int threads, double* loss, R* gvel, R* gveval, double* gvisc, double* gk) {                   \
+    oracle::State<R> S;                                                                                   \
+    load_initial(S, P, n, pos, vel, veval, clr, id, time, use_coef, visc, k);                             \
+    oracle::Adjoint<R> G;                                                                                 \
+    *loss = oracle::loss_and_grad(*P, S, nsteps, target, G, threads);                                     \
+    for (int i = 0; i < n; i++) {                                                                         \
+      gvel[3 * i] = G.gv[i].x; gvel[3 * i + 1] = G.gv[i].y; gvel[3 * i + 2] = G.gv[i].z;                  \
+      gveval[3 * i] = G.ge[i].x; gveval[3 * i + 1] = G.ge[i].y; gveval[3 * i + 2] = G.ge[i].z;            \
+    }                                                                                                     \
+    *gvisc = G.gvisc;                                                                                      \
+    *gk = G.gk;                                                                                           \
+    return 0;                                                                                             \
+  }
+ORC_GRAD(orc_grad_f32, float)
+ORC_GRAD(orc_grad_f64, double)
+
+#define ORC_LOSS(NAME, R)                                                                                 \
+  double NAME(const FParams_t* P, int n, const R* pos, const R* vel, const R* veval, const uint32_t* clr, \
+              const uint32_t* id, R time, int nsteps, int use_coef, double visc, double k, const R* target, \
+              int threads) {                                                                              \
+    oracle::State<R> S;                                                                                   \
+    load_initial(S, P, n, pos, vel, veval, clr, id, time, use_coef, visc, k);                             \
+    oracle::rollout(*P, S, nsteps, (std::vector<oracle::TapeStep<R>>*)nullptr, threads);                  \
+    oracle::Adjoint<R> G;                                                                                 \
+    return oracle::loss_seed(S, target, G);                                                               \
+  }
+ORC_LOSS(orc_loss_f32, float)
+ORC_LOSS(orc_loss_f64, double)
 
 }  // extern "C"

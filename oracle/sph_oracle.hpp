@@ -46,6 +46,11 @@ struct State {
   std::vector<uint32_t> cnt, off;  // G+1 entries
   std::vector<uint32_t> grid;      // member list of each cell (index sort)
   R time = 0;
+  // force coefficients in the state's precision when set (the FP64 rollouts
+  // of the adjoint checks perturb viscosity and stiffness below float
+  // resolution); otherwise the float FParams_t values
+  bool coef_set = false;
+  R vterm = 0, stiff = 0;
 
   void resize(int n_, int G_) {
     n = n_;
@@ -174,9 +179,10 @@ void force_one(const FParams_t& P, State<R>& S, int i) {
   if (gc == FX_GRID_UNDEF) return;
   const int base = gc - (P.gridRes.z + 1) * P.gridRes.x - 1;
   const R rd2 = P.rd2, d2 = P.d2, h = P.psmoothradius;
-  const R rest = P.prest_dens, stiff = P.pintstiff;
+  const R rest = P.prest_dens;
+  const R stiff = S.coef_set ? S.stiff : (R)P.pintstiff;
   const R kp = ((R)P.sim_scale * R(-0.5)) * (R)P.spikykern;
-  const R vterm = P.vterm;
+  const R vterm = S.coef_set ? S.vterm : (R)P.vterm;
   const V3<R> p = S.pos[i];
   const V3<R> vi = S.veval[i];
   const R di = S.press[i];
