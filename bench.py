@@ -27,6 +27,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -62,7 +64,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (200; 100 for cfg2)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg1_rest", "cfg2"])
+    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg1_rest", "cfg2", "cfg3", "cfg4"])
+    ap.add_argument("--adjoint-steps", type=int, default=20,
+                    help="cfg1: steps of the forward+adjoint rollout timed beside the plain step (0 = skip)")
+    ap.add_argument("--checkpoint-every", type=int, default=10)
     ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly")
     ap.add_argument("--stats-every", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -92,6 +97,9 @@ def main():
         sys.exit(2)
 
     from fluids3_amd import Particles, scenes
+
+    if args.workload in ("cfg3", "cfg4"):
+        return adjoint_bench(args)
 
     batch = args.workload == "cfg2"
     K = args.steps if args.steps is not None else (100 if batch else 200)
@@ -204,6 +212,11 @@ def main():
                   "batch_over_single": round(value / v1, 3),
                   "how": f"the same {len(sc)} scenes run one after another, each as a single-scene system"}
 
+    # forward+adjoint rollout beside the plain step, same start state and window
+    adjoint = None
+    if args.workload == "cfg1" and args.adjoint_steps > 0:
+        adjoint = adjoint_ratio(ps, restart, W, args.adjoint_steps, args.checkpoint_every, use_graph)
+
     cpu = None
     if want_cpu:
         from oracle import oracle
@@ -237,6 +250,107 @@ def main():
                    **({"single_scene": single} if single else {})},
         "roofline": roof,
         "cpu_baseline": cpu,
+    }
+    if adjoint is not None:
+        out["adjoint"] = adjoint
+    print(json.dumps(out))
+    ps.close()
+
+
+def adjoint_ratio(ps, restart, W, T, K, use_graph):
+    """GPU time of a T-step forward+adjoint rollout (checkpoints every K) over
+    the time of the same T plain steps, both from the state after W warm-up
+    steps.  The target is the start state's positions (the cost does not
+    depend on it)."""
+    from fluids3_amd import abi
+
+    restart()
+    ps.Run(W)
+    ps.Synchronize()
+    plain_ms, _ = ps.BenchSteps(T, use_graph=use_graph)
+    restart()
+    ps.Run(W)
+    ps.Synchronize()
+    ids = ps.Retrieve(abi.FID).astype(np.int64)
+    tgt = np.empty((ps.NumPoints(), 3), np.float32)
+    tgt[ids] = ps.Retrieve(abi.FPOS)
+    ps.AdjointConfigure(T, K)
+    ps.AdjointSetTarget(tgt)
+    _, f_ms = ps.AdjointForward()
+    g = ps.AdjointBackward()
+    tot = f_ms + g["ms"]
+    return {"steps": T, "checkpoint_every": K, "plain_ms": round(plain_ms, 4), "forward_ms": round(f_ms, 4),
+            "backward_ms": round(g["ms"], 4), "fwd_adjoint_over_plain": round(tot / plain_ms, 3),
+            "fwd_adjoint_particle_steps_per_s": round(ps.NumPoints() * T / (tot * 1e-3), 1),
+            "tape_and_checkpoint_bytes": ps.AdjointMemory()}
+
+
+def adjoint_bench(args):
+    """cfg3: one differentiable dam break; cfg4: a batch of 32.  value =
+    particle-steps/s of the forward+adjoint rollout (loss and gradients with
+    respect to initial velocities, viscosity and stiffness)."""
+    from fluids3_amd import Particles, scenes
+    from fluids3_amd.rollout import target_positions
+
+    ps = Particles()
+    if args.workload == "cfg3":
+        sc = scenes.cfg3()
+        T = args.steps if args.steps is not None else 50
+        K = args.checkpoint_every
+        target = target_positions(ps, sc, T)
+        restart = lambda: ps.Restart(sc)
+        desc = (f"cfg3: differentiable dam break, {sc.num_points:,} particles, {T} steps, checkpoints every {K}, "
+                f"target = the rollout at viscosity 0.30 and stiffness 1.2")
+    else:
+        sc = scenes.cfg4_scenes()
+        T = args.steps if args.steps is not None else 40
+        K = args.checkpoint_every if args.checkpoint_every != 10 else 8
+        # each scene fits its viscosity to the rollout at viscosity 0.35
+        target = np.concatenate([target_positions(ps, s, T, visc=0.35, k=s.int_stiff) for s in sc])
+        restart = lambda: ps.RestartBatch(sc)
+        n = [s.num_points for s in sc]
+        desc = (f"cfg4: {len(sc)} differentiable dam breaks in one batch, {sum(n):,} particles "
+                f"({min(n):,}..{max(n):,} per scene), {T} steps, checkpoints every {K}, "
+                f"target = each scene's rollout at viscosity 0.35")
+    restart()
+    N = ps.NumPoints()
+    # plain reference: the same T steps from the same start
+    ps.Run(2)
+    ps.Synchronize()
+    restart()
+    try:
+        plain_ms, _ = ps.BenchSteps(T, use_graph=not args.no_graph)
+    except Exception as e:
+        print(f"graph-timed region unavailable ({e}); timing eager launches", file=sys.stderr)
+        restart()
+        plain_ms, _ = ps.BenchSteps(T, use_graph=False)
+    restart()
+    ps.AdjointConfigure(T, K)
+    ps.AdjointSetTarget(target)
+    loss, f_ms = ps.AdjointForward()
+    g = ps.AdjointBackward()
+    tot = f_ms + g["ms"]
+    out = {
+        "metric": "forward+adjoint particle-steps/sec",
+        "value": round(N * T / (tot * 1e-3), 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": T,
+        "warmup": 0,
+        "ms_per_step": round(tot / T, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered lattice)",
+        "config": {"workload": desc, "particles": N, "checkpoint_every": K,
+                   "plain_ms": round(plain_ms, 4), "forward_ms": round(f_ms, 4), "backward_ms": round(g["ms"], 4),
+                   "fwd_adjoint_over_plain": round(tot / plain_ms, 3),
+                   "tape_and_checkpoint_bytes": ps.AdjointMemory(),
+                   "loss": [round(float(x), 6) for x in loss[:8]],
+                   "dL_dvisc": [float(x) for x in g["gvisc"][:8]], "dL_dstiff": [float(x) for x in g["gk"][:8]]},
+        "roofline": None,
+        "cpu_baseline": None,
     }
     print(json.dumps(out))
     ps.close()

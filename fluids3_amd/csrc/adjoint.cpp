@@ -1,0 +1,370 @@
+// adjoint.cpp -- reverse-mode rollouts of the SPH step (no reference
+// counterpart): checkpointed forward, recomputation per segment, adjoint
+// kernels of adjoint_gfx950.hip.
+//
+// A rollout of nsteps is cut into segments of `every` steps.  The forward
+// saves the state (pos, vel, veval, clr, id, time) before every segment and
+// after the last, and records the last segment's steps on the tape.  The
+// backward walks the segments in reverse: a segment whose steps are not on
+// the tape is recomputed from its checkpoint (the step is deterministic, so
+// the recomputed steps are bit-identical to the forward's), recording each
+// step right after its force stage; then the adjoint kernels run over the
+// recorded steps in reverse.  Memory: (segments + 1) x 44 B x N of
+// checkpoints plus every x (60 B x N + 8 B x cells) of tape.
+#include <cstring>
+#include <vector>
+
+#include "fx_common.hpp"
+#include "particles.hpp"
+
+namespace {
+
+enum { ADJ_LOSS, ADJ_ADVANCE, ADJ_FORCE, ADJ_DENSITY, ADJ_REDUCE };
+const char* kAdjNames[5] = {"adjLoss", "adjAdvance", "adjForce", "adjDensity", "adjReduce"};
+const int kAdjThreads = 256;
+
+// kernel argument block (AdjArgs in adjoint_gfx950.hip)
+struct AdjArgs {
+  const void* params;
+  const void* pbase;
+  const void* btab;
+  const void* pos;
+  const void* vel;
+  const void* veval;
+  const void* force;
+  const void* rho;
+  const void* id;
+  const void* gcell;
+  const void* cnt;
+  const void* off;
+  const void* target;
+  void* gx;
+  void* gv;
+  void* ge;
+  void* gF;
+  void* gxs;
+  void* ges;
+  void* grho;
+  void* partial;
+  float time;
+  int accumulate;
+};
+
+const size_t kCkptBytes = 44;   // pos, vel, veval (12 B each), clr, id (4 B each)
+const size_t kTapeBytes = 60;   // pos, vel, veval, force (12 B each), rho, id, gcell (4 B each)
+
+}  // namespace
+
+void Particles::AdjRelease() {
+  for (void* p : m_AdjAlloc) (void)hipFree(p);
+  m_AdjAlloc.clear();
+  m_AdjCkpt = m_AdjTape = nullptr;
+  m_AdjVec = nullptr;
+  m_AdjPartial = m_AdjAccum = nullptr;
+  m_AdjParams = m_AdjPBase = m_AdjBTab = m_AdjBFirst = nullptr;
+  m_AdjBytes = 0;
+  m_AdjTapeValid = m_AdjLossDone = false;
+  m_AdjSteps = m_AdjEvery = m_AdjSegs = m_AdjN = m_AdjCells = 0;
+}
+
+int Particles::AdjConfigure(const char* path, int nsteps, int every) {
+  if (mNumPoints <= 0) return fx::fail(FX_ESTATE, "adjoint: call Restart first");
+  if (nsteps < 0 || every < 1) return fx::fail(FX_EINVAL, "adjoint: nsteps %d, every %d", nsteps, every);
+  if (m_AdjModule == nullptr) {
+    std::string p;
+    if (path != nullptr) {
+      p = path;
+    } else {
+      const size_t k = m_CodePath.find_last_of('/');
+      p = (k == std::string::npos ? std::string() : m_CodePath.substr(0, k + 1)) + "adjoint_gfx950.hsaco";
+    }
+    FX_HIP(hipModuleLoad(&m_AdjModule, p.c_str()), "adjoint: loading the adjoint code object");
+    for (int f = 0; f < 5; f++) FX_HIP(hipModuleGetFunction(&m_AdjFunc[f], m_AdjModule, kAdjNames[f]), kAdjNames[f]);
+  }
+  FX_HIP(hipStreamSynchronize(m_Stream), "adjoint");
+  AdjRelease();
+  const int n = mNumPoints;
+  const int segs = nsteps == 0 ? 0 : (nsteps + every - 1) / every;
+  const int tape_steps = nsteps < every ? nsteps : every;
+  const size_t tape_step = kTapeBytes * (size_t)n + 8 * (size_t)m_ScanCells;
+  const int scenes = NumScenes();
+  int blocks = 0;
+  for (int s = 0; s < scenes; s++) blocks += (m_PBase[s + 1] - m_PBase[s] + kAdjThreads - 1) / kAdjThreads;
+  auto alloc = [&](size_t bytes, void** out) -> int {
+    if (bytes == 0) bytes = 8;
+    FX_HIP(hipMalloc(out, bytes), "adjoint: allocating the tape and checkpoints");
+    m_AdjAlloc.push_back(*out);
+    m_AdjBytes += (int64_t)bytes;
+    return FX_OK;
+  };
+  void* p;
+  FX_TRY(alloc(kCkptBytes * (size_t)n * (size_t)(segs + 1), &p));
+  m_AdjCkpt = static_cast<char*>(p);
+  FX_TRY(alloc(tape_step * (size_t)tape_steps, &p));
+  m_AdjTape = static_cast<char*>(p);
+  FX_TRY(alloc(sizeof(float) * (size_t)n * 22, &p));
+  m_AdjVec = static_cast<float*>(p);
+  FX_TRY(alloc(sizeof(double) * 3 * (size_t)(blocks > 0 ? blocks : 1), &p));
+  m_AdjPartial = static_cast<double*>(p);
+  FX_TRY(alloc(sizeof(double) * 3 * (size_t)scenes, &p));
+  m_AdjAccum = static_cast<double*>(p);
+  FX_TRY(alloc(sizeof(FParams_t) * (size_t)scenes, &m_AdjParams));
+  FX_TRY(alloc(sizeof(int) * (size_t)(scenes + 1), &m_AdjPBase));
+  FX_TRY(alloc(sizeof(int) * 2 * (size_t)(blocks > 0 ? blocks : 1), &m_AdjBTab));
+  FX_TRY(alloc(sizeof(int) * (size_t)(scenes + 1), &m_AdjBFirst));
+  std::vector<int> tab, first(scenes + 1, 0);
+  for (int s = 0; s < scenes; s++) {
+    first[s] = (int)tab.size() / 2;
+    const int nb = (m_PBase[s + 1] - m_PBase[s] + kAdjThreads - 1) / kAdjThreads;
+    for (int b = 0; b < nb; b++) {
+      tab.push_back(s);
+      tab.push_back(b);
+    }
+  }
+  first[scenes] = (int)tab.size() / 2;
+  if (!tab.empty()) FX_HIP(hipMemcpyHtoD(m_AdjBTab, tab.data(), tab.size() * sizeof(int)), "adjoint");
+  FX_HIP(hipMemcpyHtoD(m_AdjBFirst, first.data(), first.size() * sizeof(int)), "adjoint");
+  FX_HIP(hipMemcpyHtoD(m_AdjPBase, m_PBase.data(), sizeof(int) * (size_t)(scenes + 1)), "adjoint");
+  // target defaults to zero
+  FX_HIP(hipMemset(m_AdjVec + (size_t)n * 19, 0, sizeof(float) * 3 * (size_t)n), "adjoint");
+  m_AdjSteps = nsteps;
+  m_AdjEvery = every;
+  m_AdjSegs = segs;
+  m_AdjN = n;
+  m_AdjCells = m_ScanCells;
+  m_AdjBlocks = blocks;
+  m_AdjScenes = scenes;
+  m_AdjCkptTime.assign(segs + 1, 0.0f);
+  m_AdjTapeTime.assign(tape_steps, 0.0f);
+  return FX_OK;
+}
+
+int Particles::AdjTables() {
+  if (m_AdjN == 0 || m_AdjN != mNumPoints || m_AdjCells != m_ScanCells || m_AdjScenes != NumScenes())
+    return fx::fail(FX_ESTATE, "adjoint: configure after Restart (the particle or cell count changed)");
+  if (m_Batch) {
+    FX_HIP(hipMemcpyHtoD(m_AdjParams, m_SceneParams.data(), sizeof(FParams_t) * m_SceneParams.size()),
+           "adjoint");
+  } else {
+    FX_HIP(hipMemcpyHtoD(m_AdjParams, &m_Params, sizeof(FParams_t)), "adjoint");
+  }
+  return FX_OK;
+}
+
+int Particles::AdjSetTarget(const float* target, int64_t count) {
+  if (m_AdjN == 0) return fx::fail(FX_ESTATE, "adjoint: call fx_adjoint_configure first");
+  if (target == nullptr || count != (int64_t)m_AdjN)
+    return fx::fail(FX_EINVAL, "adjoint: the target holds one position per particle (%d)", m_AdjN);
+  FX_HIP(hipStreamSynchronize(m_Stream), "adjoint");
+  FX_HIP(hipMemcpyHtoD(m_AdjVec + (size_t)m_AdjN * 19, const_cast<float*>(target), sizeof(float) * 3 * (size_t)count),
+         "adjoint: target upload");
+  return FX_OK;
+}
+
+int Particles::AdjLaunch(int fid, void* args) {
+  if (m_AdjBlocks == 0) return FX_OK;
+  void* a[] = {args};
+  FX_HIP(hipModuleLaunchKernel(m_AdjFunc[fid], (unsigned)m_AdjBlocks, 1, 1, kAdjThreads, 1, 1, 0, m_Stream, a,
+                               nullptr),
+         kAdjNames[fid]);
+  return FX_OK;
+}
+
+int Particles::AdjSaveCheckpoint(int k) {
+  const size_t n = (size_t)m_AdjN;
+  char* c = m_AdjCkpt + kCkptBytes * n * (size_t)k;
+  const int ids[5] = {FPOS, FVEL, FVEVAL, FCLR, FID};
+  const size_t sz[5] = {12, 12, 12, 4, 4};
+  for (int b = 0; b < 5; b++) {
+    FX_HIP(hipMemcpyAsync(c, m_Points.gpu(ids[b]), sz[b] * n, hipMemcpyDeviceToDevice, m_Stream), "checkpoint");
+    c += sz[b] * n;
+  }
+  m_AdjCkptTime[k] = m_Params.time;
+  return FX_OK;
+}
+
+int Particles::AdjRestoreCheckpoint(int k) {
+  const size_t n = (size_t)m_AdjN;
+  const char* c = m_AdjCkpt + kCkptBytes * n * (size_t)k;
+  const int ids[5] = {FPOS, FVEL, FVEVAL, FCLR, FID};
+  const size_t sz[5] = {12, 12, 12, 4, 4};
+  for (int b = 0; b < 5; b++) {
+    FX_HIP(hipMemcpyAsync(m_Points.gpu(ids[b]), c, sz[b] * n, hipMemcpyDeviceToDevice, m_Stream), "restore");
+    c += sz[b] * n;
+  }
+  m_Params.time = m_AdjCkptTime[k];
+  return FX_OK;
+}
+
+// One step (Particles::Run order); with record_slot >= 0 the step's sorted
+// state after the force stage goes to that tape slot.
+int Particles::AdjStep(int record_slot) {
+  FX_TRY(InsertParticles());
+  FX_TRY(PrefixScanParticles());
+  FX_TRY(CountingSort());
+  FX_TRY(ComputePressure());
+  FX_TRY(ComputeForce());
+  if (record_slot >= 0) {
+    const size_t n = (size_t)m_AdjN;
+    char* t = m_AdjTape + (kTapeBytes * n + 8 * (size_t)m_AdjCells) * (size_t)record_slot;
+    const int ids[7] = {FPOS, FVEL, FVEVAL, FFORCE, FPRESS, FID, FGCELL};
+    const size_t sz[7] = {12, 12, 12, 12, 4, 4, 4};
+    for (int b = 0; b < 7; b++) {
+      FX_HIP(hipMemcpyAsync(t, m_Points.gpu(ids[b]), sz[b] * n, hipMemcpyDeviceToDevice, m_Stream), "tape");
+      t += sz[b] * n;
+    }
+    const size_t cb = 4 * (size_t)m_AdjCells;
+    FX_HIP(hipMemcpyAsync(t, m_Accel.gpu(AGRIDCNT), cb, hipMemcpyDeviceToDevice, m_Stream), "tape");
+    FX_HIP(hipMemcpyAsync(t + cb, m_Accel.gpu(AGRIDOFF), cb, hipMemcpyDeviceToDevice, m_Stream), "tape");
+    m_AdjTapeTime[record_slot] = m_Params.time;
+  }
+  FX_TRY(Advance());
+  AdvanceTime();
+  return FX_OK;
+}
+
+int Particles::AdjForward(double* loss, double* ms) {
+  FX_TRY(AdjTables());
+  ReleaseGraph();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms) {
+    FX_HIP(hipEventCreate(&e0), "adjoint");
+    FX_HIP(hipEventCreate(&e1), "adjoint");
+    FX_HIP(hipEventRecord(e0, m_Stream), "adjoint");
+  }
+  const int last = m_AdjSegs - 1;
+  for (int t = 0; t < m_AdjSteps; t++) {
+    const int seg = t / m_AdjEvery;
+    if (t % m_AdjEvery == 0) FX_TRY(AdjSaveCheckpoint(seg));
+    FX_TRY(AdjStep(seg == last ? t - seg * m_AdjEvery : -1));
+  }
+  FX_TRY(AdjSaveCheckpoint(m_AdjSegs));
+  // loss and seed on the final state
+  const size_t n = (size_t)m_AdjN;
+  float* V = m_AdjVec;
+  AdjArgs A = {};
+  A.params = m_AdjParams;
+  A.pbase = m_AdjPBase;
+  A.btab = m_AdjBTab;
+  A.pos = m_Points.gpu(FPOS);
+  A.id = m_Points.gpu(FID);
+  A.target = V + 19 * n;
+  A.gx = V;
+  A.gv = V + 3 * n;
+  A.ge = V + 6 * n;
+  A.partial = m_AdjPartial;
+  FX_TRY(AdjLaunch(ADJ_LOSS, &A));
+  int k0 = 0, k1 = 1, add = 0;
+  void* rargs[] = {&m_AdjPartial, &m_AdjBFirst, &m_AdjAccum, &k0, &k1, &add};
+  FX_HIP(hipModuleLaunchKernel(m_AdjFunc[ADJ_REDUCE], (unsigned)m_AdjScenes, 1, 1, kAdjThreads, 1, 1, 0, m_Stream,
+                               rargs, nullptr),
+         "adjReduce");
+  if (ms) FX_HIP(hipEventRecord(e1, m_Stream), "adjoint");
+  std::vector<double> acc(3 * (size_t)m_AdjScenes);
+  FX_HIP(hipMemcpyAsync(acc.data(), m_AdjAccum, acc.size() * sizeof(double), hipMemcpyDeviceToHost, m_Stream),
+         "adjoint");
+  FX_HIP(hipStreamSynchronize(m_Stream), "adjoint");
+  for (int s = 0; s < m_AdjScenes; s++) {
+    const int ns = m_PBase[s + 1] - m_PBase[s];
+    if (loss) loss[s] = ns > 0 ? acc[3 * s] / (double)ns : 0.0;
+  }
+  if (ms) {
+    float f = 0.0f;
+    FX_HIP(hipEventElapsedTime(&f, e0, e1), "adjoint");
+    *ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  m_AdjTapeValid = true;
+  m_AdjLossDone = true;
+  return FX_OK;
+}
+
+int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvisc, double* gk, double* ms) {
+  if (!m_AdjLossDone) return fx::fail(FX_ESTATE, "adjoint: call fx_adjoint_forward before fx_adjoint_backward");
+  FX_TRY(AdjTables());
+  ReleaseGraph();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms) {
+    FX_HIP(hipEventCreate(&e0), "adjoint");
+    FX_HIP(hipEventCreate(&e1), "adjoint");
+    FX_HIP(hipEventRecord(e0, m_Stream), "adjoint");
+  }
+  const size_t n = (size_t)m_AdjN;
+  float* V = m_AdjVec;
+  const size_t tape_step = kTapeBytes * n + 8 * (size_t)m_AdjCells;
+  bool first = true;
+  for (int seg = m_AdjSegs - 1; seg >= 0; seg--) {
+    const int s0 = seg * m_AdjEvery;
+    const int s1 = (s0 + m_AdjEvery < m_AdjSteps) ? s0 + m_AdjEvery : m_AdjSteps;
+    if (!(seg == m_AdjSegs - 1 && m_AdjTapeValid)) {
+      FX_TRY(AdjRestoreCheckpoint(seg));
+      for (int t = s0; t < s1; t++) FX_TRY(AdjStep(t - s0));
+    }
+    for (int t = s1 - 1; t >= s0; t--) {
+      const int k = t - s0;
+      const char* T = m_AdjTape + tape_step * (size_t)k;
+      AdjArgs A = {};
+      A.params = m_AdjParams;
+      A.pbase = m_AdjPBase;
+      A.btab = m_AdjBTab;
+      A.pos = T;
+      A.vel = T + 12 * n;
+      A.veval = T + 24 * n;
+      A.force = T + 36 * n;
+      A.rho = T + 48 * n;
+      A.id = T + 52 * n;
+      A.gcell = T + 56 * n;
+      A.cnt = T + 60 * n;
+      A.off = T + 60 * n + 4 * (size_t)m_AdjCells;
+      A.target = V + 19 * n;
+      A.gx = V;
+      A.gv = V + 3 * n;
+      A.ge = V + 6 * n;
+      A.gF = V + 9 * n;
+      A.gxs = V + 12 * n;
+      A.ges = V + 15 * n;
+      A.grho = V + 18 * n;
+      A.partial = m_AdjPartial;
+      A.time = m_AdjTapeTime[k];
+      A.accumulate = first ? 0 : 1;
+      FX_TRY(AdjLaunch(ADJ_ADVANCE, &A));
+      FX_TRY(AdjLaunch(ADJ_FORCE, &A));
+      FX_TRY(AdjLaunch(ADJ_DENSITY, &A));
+      first = false;
+    }
+  }
+  // the system continues from the end of the rollout
+  FX_TRY(AdjRestoreCheckpoint(m_AdjSegs));
+  m_AdjTapeValid = m_AdjSegs <= 1;
+  m_AdjLossDone = false;
+  std::vector<double> acc(3 * (size_t)m_AdjScenes, 0.0);
+  if (!first) {
+    int k0 = 1, k1 = 3, add = 0;
+    void* rargs[] = {&m_AdjPartial, &m_AdjBFirst, &m_AdjAccum, &k0, &k1, &add};
+    FX_HIP(hipModuleLaunchKernel(m_AdjFunc[ADJ_REDUCE], (unsigned)m_AdjScenes, 1, 1, kAdjThreads, 1, 1, 0,
+                                 m_Stream, rargs, nullptr),
+           "adjReduce");
+  }
+  if (ms) FX_HIP(hipEventRecord(e1, m_Stream), "adjoint");
+  if (!first)
+    FX_HIP(hipMemcpyAsync(acc.data(), m_AdjAccum, acc.size() * sizeof(double), hipMemcpyDeviceToHost, m_Stream),
+           "adjoint");
+  const size_t vb = sizeof(float) * 3 * n;
+  if (gpos) FX_HIP(hipMemcpyAsync(gpos, V, vb, hipMemcpyDeviceToHost, m_Stream), "adjoint");
+  if (gvel) FX_HIP(hipMemcpyAsync(gvel, V + 3 * n, vb, hipMemcpyDeviceToHost, m_Stream), "adjoint");
+  if (gveval) FX_HIP(hipMemcpyAsync(gveval, V + 6 * n, vb, hipMemcpyDeviceToHost, m_Stream), "adjoint");
+  FX_HIP(hipStreamSynchronize(m_Stream), "adjoint");
+  for (int s = 0; s < m_AdjScenes; s++) {
+    if (gk) gk[s] = acc[3 * s + 1];
+    if (gvisc) gvisc[s] = acc[3 * s + 2];
+  }
+  if (ms) {
+    float f = 0.0f;
+    FX_HIP(hipEventElapsedTime(&f, e0, e1), "adjoint");
+    *ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  return FX_OK;
+}

@@ -1,0 +1,413 @@
+// adjoint_gfx950.hip -- reverse-mode (adjoint) kernels of the SPH step for
+// gfx950 (CDNA4, wave64).  No reference counterpart: the reference has no
+// differentiable path.  They reverse computePressure, computeForce and
+// advanceParticles (fluids5.0/src/particles.cu:120-157, 188-225, 237-322) as
+// implemented in particles_gfx950.hip, over a recorded step: the step's
+// cell-sorted positions, velocities, evaluation velocities, forces,
+// densities, ids and cells and its count/offset tables.  Neighbour sets,
+// cells, branch masks and the sort permutation are taken from that forward
+// step as piecewise-constant data.
+//
+// Adjoint variables of the state live in the particle-id frame (index =
+// original particle index): reading and writing them through the recorded
+// ids undoes each step's sort.  Per step, for the step's sorted slots:
+//   adjAdvance  (gx', gv', ge') of the outputs -> gF, partial gx/ge of the
+//               inputs (sorted frame), gv of the velocity input (id frame);
+//   adjForce    gather over in-radius neighbours j (dl = gF_i - gF_j,
+//               u = x_i - x_j, dve = ve_j - ve_i, inv = 1/(r rho_i rho_j)):
+//                 gx_i   += A dl + (d2/r) u (A_r (u.dl) + B_r (dve.dl))
+//                 ge_i   -= B dl
+//                 grho_i += kp c^2 inv k (u.dl) - (A (u.dl) + B (dve.dl)) / rho_i
+//               and the per-particle terms of dL/dk and dL/dvisc;
+//   adjDensity  gx_i += sum_j (grho_i + grho_j) (-6 m poly6 d2) t^2 u, then
+//               the step's input adjoints go back to the id frame.
+// Every pair term is gathered from i's side (A and B are symmetric in i, j),
+// so there are no atomics and results are reproducible bit for bit.  FP64 is
+// used only for the loss and the two scalar-parameter sums: per-block FP64
+// partials, reduced per scene in a fixed order.
+//
+// Arithmetic contract as in particles_gfx950.hip: -ffp-contract=off, explicit
+// fmaf, IEEE sqrt and division; the CPU adjoint oracle evaluates the same
+// trees.
+#include <hip/hip_runtime.h>
+
+#include "fluidsx_params.h"
+
+typedef unsigned int uint;
+
+#define ADJ_THREADS 256
+
+template <class T>
+__device__ __forceinline__ T* gptr(T* p) {
+  typedef __attribute__((address_space(1))) T gT;
+  return (T*)(gT*)p;
+}
+
+__device__ __forceinline__ const FParams_t* cparams(const FParams_t* p) {
+  typedef const __attribute__((address_space(4))) FParams_t cP;
+  return (const FParams_t*)(cP*)p;
+}
+
+// Everything one adjoint launch reads or writes.  Particle vectors are fx_float3
+// arrays; "sorted" arrays are indexed by the step's slot, "id" arrays by
+// particle id.
+struct AdjArgs {
+  const FParams_t* params;  // one per scene
+  const int* pbase;         // first particle of each scene (nscene + 1)
+  const int2* btab;         // (scene, block inside the scene) per launched block
+  // recorded forward step (sorted frame)
+  const fx_float3* pos;
+  const fx_float3* vel;
+  const fx_float3* veval;
+  const fx_float3* force;
+  const float* rho;
+  const uint* id;
+  const int* gcell;
+  const uint* cnt;
+  const uint* off;
+  const fx_float3* target;  // id frame (loss only)
+  // adjoint state (id frame)
+  fx_float3* gx;
+  fx_float3* gv;
+  fx_float3* ge;
+  // per-step work (sorted frame)
+  fx_float3* gF;
+  fx_float3* gxs;
+  fx_float3* ges;
+  float* grho;
+  double* partial;  // 3 doubles per launched block: loss, dL/dk, dL/dvisc
+  float time;
+  int accumulate;  // adjForce: add to the block's partials instead of overwriting
+};
+
+struct Slot {
+  const FParams_t* P;
+  int i, p0, p1, scene;
+};
+
+__device__ __forceinline__ Slot slot_of(const AdjArgs& A) {
+  const int2 e = A.btab[blockIdx.x];
+  Slot s;
+  s.P = cparams(A.params + e.x);
+  s.scene = e.x;
+  s.p0 = A.pbase[e.x];
+  s.p1 = A.pbase[e.x + 1];
+  s.i = s.p0 + e.y * ADJ_THREADS + (int)threadIdx.x;
+  return s;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block sum of one double per thread in a fixed order (wave butterflies, then
+// waves in index order); thread 0 gets the result.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  v = wave_sum(v);
+  const int w = (int)threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < ADJ_THREADS / 64; k++) s += lds[k];
+  __syncthreads();
+  return s;
+}
+
+// ------------------------------------------------------------------- loss
+// L_s = (1/n_s) sum |x - x*|^2 over scene s (FP64), seed gx[id] = (2/n_s)(x - x*),
+// gv = ge = 0.  pos/id: the final state in its slot order.
+extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjLoss(AdjArgs A) {
+  __shared__ double lds[ADJ_THREADS / 64];
+  const Slot S = slot_of(A);
+  double l = 0.0;
+  if (S.i < S.p1) {
+    const uint id = A.id[S.i];
+    const fx_float3 x = A.pos[S.i], t = A.target[id];
+    const float scale = 2.0f / (float)(S.p1 - S.p0);
+    const float dx = x.x - t.x, dy = x.y - t.y, dz = x.z - t.z;
+    l = (double)dx * (double)dx + (double)dy * (double)dy + (double)dz * (double)dz;
+    A.gx[id] = fx_float3{dx * scale, dy * scale, dz * scale};
+    A.gv[id] = fx_float3{0.0f, 0.0f, 0.0f};
+    A.ge[id] = fx_float3{0.0f, 0.0f, 0.0f};
+  }
+  l = block_sum(l, lds);
+  if (threadIdx.x == 0) A.partial[3 * blockIdx.x] = l;
+}
+
+// ---------------------------------------------------------------- advance
+extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) {
+  const Slot S = slot_of(A);
+  if (S.i >= S.p1) return;
+  const FParams_t& P = *S.P;
+  const int s = S.i;
+  const uint id = A.id[s];
+  const fx_float3 gxo = A.gx[id], gvo = A.gv[id], geo = A.ge[id];
+  if (A.gcell[s] == FX_GRID_UNDEF) {  // parked: pos and vel become constants, veval passes through
+    const fx_float3 z = {0.0f, 0.0f, 0.0f};
+    A.gF[s] = z;
+    A.gxs[s] = z;
+    A.ges[s] = geo;
+    A.gv[id] = z;
+    return;
+  }
+  const fx_float3 pos = A.pos[s], vel = A.vel[s], ve = A.veval[s], f = A.force[s];
+  const float m = P.pmass, dt = P.dt, ss = P.sim_scale;
+  float ax = f.x * m, ay = f.y * m, az = f.z * m;
+  const float rad = P.pradius, stiff = P.bound_stiff, damp = P.bound_damp;
+  const float slope = P.bound_slope, fric = P.bound_friction;
+  const fx_float3 bmin = P.bound_min, bmax = P.bound_max;
+  float diff, adj, nx, ny, nz;
+  // forward replay (advance_one of particles_gfx950.hip): pre-limit
+  // acceleration and the active walls
+  diff = rad - (pos.y - (bmin.y + (pos.x - bmin.x) * slope)) * ss;
+  const bool w1 = diff > FX_EPSILON;
+  if (w1) {
+    nx = -slope; ny = 1.0f - slope; nz = 0.0f;
+    adj = stiff * diff - damp * (nx * ve.x + ny * ve.y + nz * ve.z);
+    nx *= adj; ny *= adj; nz *= adj;
+    ax += nx - ve.x * fric; ay += ny - ve.y * fric; az += nz - ve.z * fric;
+  }
+  diff = rad - (bmax.y - pos.y) * ss;
+  const bool w2 = diff > FX_EPSILON;
+  if (w2) {
+    nx = 0.0f; ny = -1.0f; nz = 0.0f;
+    adj = stiff * diff - damp * (nx * ve.x + ny * ve.y + nz * ve.z);
+    nx *= adj; ny *= adj; nz *= adj;
+    ax += nx - ve.x * fric; ay += ny - ve.y * fric; az += nz - ve.z * fric;
+  }
+  const float sarg = A.time * P.bound_wall_freq + pos.z / 400.0f;
+  const float wall = (float)(((double)sinf(sarg) * 0.5 + 0.5) * (double)P.bound_wall_force);
+  diff = rad - (pos.x - (bmin.x + wall)) * ss;
+  const float diff3 = diff;
+  const bool w3 = diff > FX_EPSILON;
+  if (w3) {
+    nx = 1.0f; ny = 0.0f; nz = 0.0f;
+    adj = wall * stiff * diff - damp * (nx * ve.x + ny * ve.y + nz * ve.z);
+    ax += nx * adj; ay += ny * adj; az += nz * adj;
+  }
+  diff = rad - (bmax.x - pos.x) * ss;
+  const bool w4 = diff > FX_EPSILON;
+  if (w4) {
+    nx = -1.0f; ny = 0.0f; nz = 0.0f;
+    adj = stiff * diff - damp * (nx * ve.x + ny * ve.y + nz * ve.z);
+    ax += nx * adj; ay += ny * adj; az += nz * adj;
+  }
+  diff = rad - (pos.z - bmin.z) * ss;
+  const bool w5 = diff > FX_EPSILON;
+  if (w5) {
+    nx = 0.0f; ny = 0.0f; nz = 1.0f;
+    adj = stiff * diff - damp * (nx * ve.x + ny * ve.y + nz * ve.z);
+    ax += nx * adj; ay += ny * adj; az += nz * adj;
+  }
+  diff = rad - (bmax.z - pos.z) * ss;
+  const bool w6 = diff > FX_EPSILON;
+  if (w6) {
+    nx = 0.0f; ny = 0.0f; nz = -1.0f;
+    adj = stiff * diff - damp * (nx * ve.x + ny * ve.y + nz * ve.z);
+    ax += nx * adj; ay += ny * adj; az += nz * adj;
+  }
+  ax += P.gravity.x;
+  ay += P.gravity.y;
+  az += P.gravity.z;
+  const float sa2 = ax * ax + ay * ay + az * az;
+  const float sv2 = vel.x * vel.x + vel.y * vel.y + vel.z * vel.z;
+
+  // reverse of x' = x + vn dt/ss, v' = vn, ve' = (v_l + vn)/2, vn = a_l dt + v_l
+  const float dts = dt / ss;
+  const float gnx = gxo.x * dts + gvo.x + geo.x * 0.5f;
+  const float gny = gxo.y * dts + gvo.y + geo.y * 0.5f;
+  const float gnz = gxo.z * dts + gvo.z + geo.z * 0.5f;
+  float gvx = gnx + geo.x * 0.5f, gvy = gny + geo.y * 0.5f, gvz = gnz + geo.z * 0.5f;
+  float gax = gnx * dt, gay = gny * dt, gaz = gnz * dt;
+  if (sv2 > P.VL2) {  // v_l = v VL/|v|: J = (VL/|v|)(I - v v^T/|v|^2)
+    const float sv = P.VL / sqrtf(sv2);
+    const float d = (vel.x * gvx + vel.y * gvy + vel.z * gvz) / sv2;
+    gvx = (gvx - vel.x * d) * sv;
+    gvy = (gvy - vel.y * d) * sv;
+    gvz = (gvz - vel.z * d) * sv;
+  }
+  if (sa2 > P.AL2) {
+    const float sa = P.AL / sqrtf(sa2);
+    const float d = (ax * gax + ay * gay + az * gaz) / sa2;
+    gax = (gax - ax * d) * sa;
+    gay = (gay - ay * d) * sa;
+    gaz = (gaz - az * d) * sa;
+  }
+  A.gv[id] = fx_float3{gvx, gvy, gvz};
+  A.gF[s] = fx_float3{gax * m, gay * m, gaz * m};
+  float px = gxo.x, py = gxo.y, pz = gxo.z;
+  float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+  if (w1) {
+    nx = -slope; ny = 1.0f - slope;
+    const float dn = nx * gax + ny * gay;
+    const float t = stiff * dn * ss;
+    px += t * slope;
+    py -= t;
+    const float dd = damp * dn;
+    ex -= dd * nx + fric * gax;
+    ey -= dd * ny + fric * gay;
+    ez -= fric * gaz;
+  }
+  if (w2) {
+    const float dn = -gay;
+    py += stiff * dn * ss;
+    ex -= fric * gax;
+    ey -= fric * gay - damp * dn;
+    ez -= fric * gaz;
+  }
+  if (w3) {
+    px -= gax * (wall * stiff * ss);
+    if (P.bound_wall_force != 0.0f) {
+      const float dwall = cosf(sarg) * (0.5f / 400.0f) * P.bound_wall_force;
+      pz += gax * (stiff * dwall * (diff3 + wall * ss));
+    }
+    ex -= damp * gax;
+  }
+  if (w4) {
+    px -= gax * (stiff * ss);
+    ex -= damp * gax;
+  }
+  if (w5) {
+    pz -= gaz * (stiff * ss);
+    ez -= damp * gaz;
+  }
+  if (w6) {
+    pz -= gaz * (stiff * ss);
+    ez -= damp * gaz;
+  }
+  A.gxs[s] = fx_float3{px, py, pz};
+  A.ges[s] = fx_float3{ex, ey, ez};
+}
+
+// ------------------------------------------------------------------ force
+extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjForce(AdjArgs A) {
+  __shared__ double lds[ADJ_THREADS / 64];
+  const Slot S = slot_of(A);
+  const FParams_t& P = *S.P;
+  float ak = 0.0f, av = 0.0f;
+  const int i = S.i;
+  if (i < S.p1 && A.gcell[i] != FX_GRID_UNDEF) {
+    const int gc = A.gcell[i];
+    const int base = gc - (P.gridRes.z + 1) * P.gridRes.x - 1;
+    const int srch = P.gridSrch;
+    const float rd2 = P.rd2, d2 = P.d2, h = P.psmoothradius, rest = P.prest_dens;
+    const float stiff = P.pintstiff, vterm = P.vterm, lap = P.lapkern;
+    const float kp = (P.sim_scale * -0.5f) * P.spikykern;
+    const fx_float3 p = A.pos[i], vi = A.veval[i], gi = A.gF[i];
+    const float di = A.rho[i];
+    const float pi = (di - rest) * stiff;
+    const float idi = 1.0f / di;
+    const float qi = di - rest;
+    float gpx = 0.0f, gpy = 0.0f, gpz = 0.0f, gex = 0.0f, gey = 0.0f, gez = 0.0f, gr = 0.0f;
+    for (int r = 0; r < srch * srch; r++) {
+      const int c0 = base + P.gridAdj[r * srch];
+      const int c1 = c0 + srch - 1;
+      const int jb = (int)A.off[c0], je = (int)(A.off[c1] + A.cnt[c1]);
+      for (int j = jb; j < je; j++) {
+        const fx_float3 q = A.pos[j];
+        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (!(dsq < rd2 && dsq > 0.0f)) continue;
+        const float rr = sqrtf(dsq * d2);
+        const float c = h - rr;
+        const float dj = A.rho[j];
+        const float pj = (dj - rest) * stiff;
+        const float inv = 1.0f / (rr * (di * dj));
+        const float invd = inv * rr;
+        const float ir = inv * (di * dj);
+        const float kc2i = (kp * (c * c)) * inv;
+        const float pp = pi + pj;
+        const float Ac = kc2i * pp;
+        const float Bc = (vterm * c) * invd;
+        const fx_float3 gj = A.gF[j], vj = A.veval[j];
+        const float lx = gi.x - gj.x, ly = gi.y - gj.y, lz = gi.z - gj.z;
+        const float wx = vj.x - vi.x, wy = vj.y - vi.y, wz = vj.z - vi.z;
+        const float su = fmaf(dz, lz, fmaf(dy, ly, dx * lx));
+        const float sv = fmaf(wz, lz, fmaf(wy, ly, wx * lx));
+        const float Ar = -(kp * pp) * (c * (rr + rr + c)) * (invd * (ir * ir));
+        const float Br = -vterm * invd;
+        const float g = (d2 * ir) * fmaf(Ar, su, Br * sv);
+        gpx = fmaf(Ac, lx, fmaf(g, dx, gpx));
+        gpy = fmaf(Ac, ly, fmaf(g, dy, gpy));
+        gpz = fmaf(Ac, lz, fmaf(g, dz, gpz));
+        gex = fmaf(-Bc, lx, gex);
+        gey = fmaf(-Bc, ly, gey);
+        gez = fmaf(-Bc, lz, gez);
+        gr = fmaf(kc2i * stiff, su, gr) - fmaf(Ac, su, Bc * sv) * idi;
+        const float ui = fmaf(dz, gi.z, fmaf(dy, gi.y, dx * gi.x));
+        const float wi = fmaf(wz, gi.z, fmaf(wy, gi.y, wx * gi.x));
+        ak = fmaf(kc2i * (qi + (dj - rest)), ui, ak);
+        av = fmaf((lap * c) * invd, wi, av);
+      }
+    }
+    const fx_float3 x0 = A.gxs[i], e0 = A.ges[i];
+    A.gxs[i] = fx_float3{x0.x + gpx, x0.y + gpy, x0.z + gpz};
+    A.ges[i] = fx_float3{e0.x + gex, e0.y + gey, e0.z + gez};
+    A.grho[i] = gr;
+  } else if (i < S.p1) {
+    A.grho[i] = 0.0f;
+  }
+  const double sk = block_sum((double)ak, lds);
+  const double sv = block_sum((double)av, lds);
+  if (threadIdx.x == 0) {
+    double* o = A.partial + 3 * blockIdx.x;
+    o[1] = A.accumulate ? o[1] + sk : sk;
+    o[2] = A.accumulate ? o[2] + sv : sv;
+  }
+}
+
+// ---------------------------------------------------------------- density
+extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjDensity(AdjArgs A) {
+  const Slot S = slot_of(A);
+  const int i = S.i;
+  if (i >= S.p1) return;
+  const FParams_t& P = *S.P;
+  float gpx = 0.0f, gpy = 0.0f, gpz = 0.0f;
+  if (A.gcell[i] != FX_GRID_UNDEF) {
+    const int gc = A.gcell[i];
+    const int base = gc - (P.gridRes.z + 1) * P.gridRes.x - 1;
+    const int srch = P.gridSrch;
+    const float rd2 = P.rd2, d2 = P.d2;
+    const float k6 = ((-6.0f * P.pmass) * P.poly6kern) * d2;
+    const fx_float3 p = A.pos[i];
+    const float gi = A.grho[i];
+    for (int r = 0; r < srch * srch; r++) {
+      const int c0 = base + P.gridAdj[r * srch];
+      const int c1 = c0 + srch - 1;
+      const int jb = (int)A.off[c0], je = (int)(A.off[c1] + A.cnt[c1]);
+      for (int j = jb; j < je; j++) {
+        const fx_float3 q = A.pos[j];
+        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (!(dsq < rd2 && dsq > 0.0f)) continue;
+        const float t = (rd2 - dsq) * d2;
+        const float cf = ((gi + A.grho[j]) * (t * t)) * k6;
+        gpx = fmaf(cf, dx, gpx);
+        gpy = fmaf(cf, dy, gpy);
+        gpz = fmaf(cf, dz, gpz);
+      }
+    }
+  }
+  const uint id = A.id[i];
+  const fx_float3 x0 = A.gxs[i];
+  A.gx[id] = fx_float3{x0.x + gpx, x0.y + gpy, x0.z + gpz};
+  A.ge[id] = A.ges[i];
+}
+
+// ----------------------------------------------------------------- reduce
+// accum[3 s + k] (+)= sum of partial[3 b + k] over scene s's blocks
+// [bfirst[s], bfirst[s+1]), k in [k0, k1), in a fixed order.  One block per scene.
+extern "C" __global__ void __launch_bounds__(ADJ_THREADS)
+    adjReduce(const double* partial, const int* bfirst, double* accum, int k0, int k1, int add) {
+  __shared__ double lds[ADJ_THREADS / 64];
+  const int s = blockIdx.x;
+  const int b0 = bfirst[s], b1 = bfirst[s + 1];
+  for (int k = k0; k < k1; k++) {
+    double v = 0.0;
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += ADJ_THREADS) v += partial[3 * b + k];
+    v = block_sum(v, lds);
+    if (threadIdx.x == 0) accum[3 * s + k] = add ? accum[3 * s + k] + v : v;
+  }
+}

@@ -189,6 +189,29 @@ int fx_set_block_size(fx_system* s, int32_t threads) {
   return s->p.SetBlockSize(threads);
 }
 
+int fx_adjoint_configure(fx_system* s, const char* path, int32_t nsteps, int32_t every) {
+  if (s == nullptr) return fx::fail(FX_EINVAL, "null system");
+  return s->p.AdjConfigure(path, nsteps, every);
+}
+
+int fx_adjoint_set_target(fx_system* s, const float* target, int64_t count) {
+  if (s == nullptr) return fx::fail(FX_EINVAL, "null system");
+  return s->p.AdjSetTarget(target, count);
+}
+
+int fx_adjoint_forward(fx_system* s, double* loss, double* ms) {
+  if (s == nullptr) return fx::fail(FX_EINVAL, "null system");
+  return s->p.AdjForward(loss, ms);
+}
+
+int fx_adjoint_backward(fx_system* s, float* gpos, float* gvel, float* gveval, double* gvisc, double* gk,
+                        double* ms) {
+  if (s == nullptr) return fx::fail(FX_EINVAL, "null system");
+  return s->p.AdjBackward(gpos, gvel, gveval, gvisc, gk, ms);
+}
+
+int64_t fx_adjoint_memory(const fx_system* s) { return s == nullptr ? 0 : s->p.AdjMemory(); }
+
 const char* fx_last_error(void) { return fx::last_error().c_str(); }
 
 const char* fx_version(void) { return "fluidsx 0.1 gfx950"; }

@@ -35,6 +35,8 @@ static void free_batch(void*& a, void*& b, void*& c, std::map<int, std::pair<voi
 
 Particles::~Particles() {
   ReleaseGraph();
+  AdjRelease();
+  if (m_AdjModule) (void)hipModuleUnload(m_AdjModule);
   free_batch(m_DParams, m_DPBase, m_DCBase, m_BTab);
   m_Points.DeleteAllBuffers();
   m_PointsTemp.DeleteAllBuffers();
@@ -49,6 +51,7 @@ Particles::~Particles() {
 int Particles::Initialize(const char* path) {
   if (m_Module != nullptr) return fx::fail(FX_ESTATE, "Initialize: already initialized");
   FX_HIP(hipModuleLoad(&m_Module, path), "Initialize");
+  m_CodePath = path;
   for (int f = 0; f < FUNC_MAX; f++)
     FX_HIP(hipModuleGetFunction(&m_Func[f], m_Module, kKernelNames[f]), kKernelNames[f]);
   FX_TRY(m_Points.AssignToGPU("FPnts", m_Module));
@@ -85,6 +88,7 @@ int Particles::Restart(const fx_scene& sc) {
   if (m_Module == nullptr) return fx::fail(FX_ESTATE, "Restart: call Initialize first");
   FX_HIP(hipStreamSynchronize(m_Stream), "Restart");
   ReleaseGraph();
+  AdjRelease();
   free_batch(m_DParams, m_DPBase, m_DCBase, m_BTab);
   m_Batch = false;
   m_SceneParams.clear();
@@ -115,6 +119,7 @@ int Particles::RestartBatch(const fx_scene* scenes, int count) {
   if (scenes == nullptr || count <= 0) return fx::fail(FX_EINVAL, "RestartBatch: no scenes");
   FX_HIP(hipStreamSynchronize(m_Stream), "RestartBatch");
   ReleaseGraph();
+  AdjRelease();
   free_batch(m_DParams, m_DPBase, m_DCBase, m_BTab);
   std::vector<FParams_t> ps(count);
   std::vector<int> pb(count + 1, 0), cb(count + 1, 0);
@@ -688,12 +693,14 @@ int Particles::BenchSteps(int nsteps, bool use_graph, double* ms_total, double* 
       for (int s = 0; s < nsteps && st == FX_OK; s++)
         for (int k = 0; k < FX_NUM_STAGES && st == FX_OK; k++) {
           // an external event node, so that the replay records a timestamp
-          if (hipEventRecordWithFlags(ev[s * FX_NUM_STAGES + k], m_Stream, hipEventRecordExternal) != hipSuccess)
-            st = fx::fail(FX_EHIP, "bench: event record in capture");
+          const hipError_t er = hipEventRecordWithFlags(ev[s * FX_NUM_STAGES + k], m_Stream, hipEventRecordExternal);
+          if (er != hipSuccess) st = fx::hip_fail(er, "bench", "event record in capture");
           if (st == FX_OK) st = (k == FX_STAGE_ADVANCE) ? Advance() : Stage(k);
         }
-      if (st == FX_OK && hipEventRecordWithFlags(ev[nev - 1], m_Stream, hipEventRecordExternal) != hipSuccess)
-        st = fx::fail(FX_EHIP, "bench: event record in capture");
+      if (st == FX_OK) {
+        const hipError_t er = hipEventRecordWithFlags(ev[nev - 1], m_Stream, hipEventRecordExternal);
+        if (er != hipSuccess) st = fx::hip_fail(er, "bench", "last event record in capture");
+      }
       m_Capturing = false;
       hipError_t e = hipStreamEndCapture(m_Stream, &g);
       if (st == FX_OK && e != hipSuccess) st = fx::hip_fail(e, "bench", "hipStreamEndCapture");

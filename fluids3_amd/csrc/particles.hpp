@@ -40,6 +40,7 @@
 #define FUNC_MAX 19
 
 #include <map>
+#include <string>
 #include <vector>
 
 // Kernel argument block of the *Batch kernels (BatchArgs in the kernel file).
@@ -104,6 +105,13 @@ class Particles {
   int BenchSteps(int nsteps, bool use_graph, double* ms_total, double* stage_ms);
   int CountPairs(int64_t* cand, int64_t* pairs);
 
+  // reverse mode (adjoint.cpp): checkpointed rollouts and their gradients
+  int AdjConfigure(const char* code_object_path, int nsteps, int every);
+  int AdjSetTarget(const float* target, int64_t count);
+  int AdjForward(double* loss, double* ms);
+  int AdjBackward(float* gpos, float* gvel, float* gveval, double* gvisc, double* gk, double* ms);
+  int64_t AdjMemory() const { return m_AdjBytes; }
+
   int NumPoints() const { return mNumPoints; }
   int CellTableSize() const { return m_ScanCells; }
   const FParams_t& Params() const { return m_Params; }
@@ -119,6 +127,13 @@ class Particles {
   int BuildBlockTables();
   FxBatchArgs BatchFor(int stage, unsigned* blocks) const;
   int FinishRestart(int n);
+  // adjoint helpers
+  void AdjRelease();
+  int AdjTables();
+  int AdjStep(int record_slot);
+  int AdjSaveCheckpoint(int k);
+  int AdjRestoreCheckpoint(int k);
+  int AdjLaunch(int fid, void* args);
 
   hipModule_t m_Module = nullptr;
   hipFunction_t m_Func[FUNC_MAX] = {};
@@ -156,4 +171,25 @@ class Particles {
   void* m_DPBase = nullptr;
   void* m_DCBase = nullptr;
   std::map<int, std::pair<void*, int>> m_BTab;  // threads per block -> (int2 table, blocks)
+
+  // adjoint (reverse mode)
+  std::string m_CodePath;
+  hipModule_t m_AdjModule = nullptr;
+  hipFunction_t m_AdjFunc[5] = {};
+  int m_AdjSteps = 0, m_AdjEvery = 0, m_AdjSegs = 0, m_AdjN = 0, m_AdjCells = 0;
+  bool m_AdjTapeValid = false, m_AdjLossDone = false;
+  int64_t m_AdjBytes = 0;
+  std::vector<void*> m_AdjAlloc;         // every device allocation of the adjoint
+  std::vector<float> m_AdjCkptTime;      // checkpoint k = state before step k*every
+  std::vector<float> m_AdjTapeTime;      // time of each recorded step of the segment
+  char* m_AdjCkpt = nullptr;             // (segs+1) x 44 B x N
+  char* m_AdjTape = nullptr;             // every x (60 B x N + 8 B x cells)
+  float* m_AdjVec = nullptr;             // gx, gv, ge, gF, gxs, ges (3N floats each), grho (N), target (3N)
+  double* m_AdjPartial = nullptr;        // 3 per block
+  double* m_AdjAccum = nullptr;          // 3 per scene
+  void* m_AdjParams = nullptr;           // FParams_t per scene
+  void* m_AdjPBase = nullptr;
+  void* m_AdjBTab = nullptr;             // int2 per block
+  void* m_AdjBFirst = nullptr;           // first block of each scene (scenes + 1)
+  int m_AdjBlocks = 0, m_AdjScenes = 0;
 };

@@ -54,6 +54,12 @@ SIGNATURES = {
     "fx_set_block_size": (C.c_int, [_P, C.c_int32]),
     "fx_set_stage_block_size": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "fx_stage_block_size": (C.c_int32, [_P, C.c_int32]),
+    "fx_adjoint_configure": (C.c_int, [_P, C.c_char_p, C.c_int32, C.c_int32]),
+    "fx_adjoint_set_target": (C.c_int, [_P, _P, C.c_int64]),
+    "fx_adjoint_forward": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "fx_adjoint_backward": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]),
+    "fx_adjoint_memory": (C.c_int64, [_P]),
     "fx_last_error": (C.c_char_p, []),
     "fx_version": (C.c_char_p, []),
 }
@@ -277,6 +283,42 @@ class Particles:
         c, p = C.c_int64(), C.c_int64()
         check(lib().fx_count_pairs(self._h, C.byref(c), C.byref(p)), "fx_count_pairs")
         return c.value, p.value
+
+    # ---- reverse mode
+    def AdjointConfigure(self, nsteps, every, code_object=None):
+        """Checkpoints every `every` steps of an nsteps rollout from the current state."""
+        path = None if code_object is None else code_object.encode()
+        check(lib().fx_adjoint_configure(self._h, path, nsteps, every), "fx_adjoint_configure")
+
+    def AdjointSetTarget(self, target):
+        """Target positions, (n, 3) float32 in particle-id order."""
+        t = np.ascontiguousarray(target, dtype=np.float32).reshape(-1, 3)
+        check(lib().fx_adjoint_set_target(self._h, t.ctypes.data, t.shape[0]), "fx_adjoint_set_target")
+
+    def AdjointForward(self):
+        """(loss per scene as float64 array, GPU ms) of the configured rollout."""
+        loss = (C.c_double * self.NumScenes())()
+        ms = C.c_double()
+        check(lib().fx_adjoint_forward(self._h, loss, C.byref(ms)), "fx_adjoint_forward")
+        return np.array(loss[:], dtype=np.float64), ms.value
+
+    def AdjointBackward(self):
+        """Gradients of the last AdjointForward's loss: dict with gpos, gvel, gveval ((n, 3) float32 by
+        particle id), gvisc, gk (float64 per scene) and ms (GPU time)."""
+        n, ns = self.NumPoints(), self.NumScenes()
+        gp = np.empty((n, 3), np.float32)
+        gv = np.empty((n, 3), np.float32)
+        ge = np.empty((n, 3), np.float32)
+        gvisc = (C.c_double * ns)()
+        gk = (C.c_double * ns)()
+        ms = C.c_double()
+        check(lib().fx_adjoint_backward(self._h, gp.ctypes.data, gv.ctypes.data, ge.ctypes.data, gvisc, gk,
+                                        C.byref(ms)), "fx_adjoint_backward")
+        return {"gpos": gp, "gvel": gv, "gveval": ge, "gvisc": np.array(gvisc[:]), "gk": np.array(gk[:]),
+                "ms": ms.value}
+
+    def AdjointMemory(self):
+        return lib().fx_adjoint_memory(self._h)
 
     def TimeSteps(self, nsteps, use_graph=True):
         ms = C.c_double()

@@ -99,4 +99,14 @@ def batch_scenes(count=64, lo=8192, hi=262144, seed=SEED, visc_range=(0.2, 0.5))
     return [scaled_dam_break(int(n), seed=seed + 1 + k, visc=float(v)) for k, (n, v) in enumerate(zip(ns, viscs))]
 
 
-PRESETS = {"cfg0": cfg0, "cfg1": cfg1, "cfg1_rest": cfg1_rest}
+def cfg3(n=262144, seed=SEED):
+    """Differentiable dam break: 262,144 particles at rest spacing, fill 0.4."""
+    return scaled_dam_break(n, seed=seed)
+
+
+def cfg4_scenes(count=32, seed=SEED):
+    """Batched adjoint: counts log-uniform in [16,384, 1,048,576], viscosity uniform in [0.2, 0.5]."""
+    return batch_scenes(count, lo=16384, hi=1048576, seed=seed)
+
+
+PRESETS = {"cfg0": cfg0, "cfg1": cfg1, "cfg1_rest": cfg1_rest, "cfg3": cfg3}

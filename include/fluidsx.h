@@ -176,6 +176,38 @@ int fx_set_block_size(fx_system* sys, int32_t threads);
 int fx_set_stage_block_size(fx_system* sys, int32_t stage, int32_t threads);
 int32_t fx_stage_block_size(const fx_system* sys, int32_t stage);
 
+/* ---- reverse mode (no reference counterpart: the reference has no
+ * differentiable path) ----
+ *
+ * Loss of an nsteps rollout from the current state, L_s = (1/n_s) sum over
+ * scene s of |x_T - x*|^2 (FP64 reduction), and its gradient with respect to
+ * the initial positions, velocities and evaluation velocities of every
+ * particle (by particle id, FID) and to each scene's viscosity and internal
+ * stiffness.  Adjoint kernels of density, force and advance run over the
+ * recorded forward steps; neighbour sets, cells, wall and limiter branches
+ * and the sort permutation of the forward are held fixed.  State is
+ * checkpointed every `checkpoint_every` steps and each segment is recomputed
+ * once in the backward pass. */
+
+/* Loads the adjoint code object (NULL: adjoint_gfx950.hsaco beside the step's
+ * code object) and allocates checkpoints and tape for the current Restart. */
+int fx_adjoint_configure(fx_system* sys, const char* code_object_path, int32_t nsteps,
+                         int32_t checkpoint_every);
+/* Target positions x*, 3 floats per particle in particle-id order (count =
+ * particles).  Defaults to zero. */
+int fx_adjoint_set_target(fx_system* sys, const float* target, int64_t count);
+/* Runs the rollout (the system ends at step nsteps), records checkpoints and
+ * the last segment's steps, writes the loss of every scene (fx_num_scenes
+ * entries).  ms (optional) receives the GPU time of the call. */
+int fx_adjoint_forward(fx_system* sys, double* loss, double* ms);
+/* Gradients of the last fx_adjoint_forward's loss (each output optional):
+ * grad_pos / grad_vel / grad_veval 3 floats per particle by id, grad_visc /
+ * grad_stiff one double per scene.  The system is left at step nsteps. */
+int fx_adjoint_backward(fx_system* sys, float* grad_pos, float* grad_vel, float* grad_veval, double* grad_visc,
+                        double* grad_stiff, double* ms);
+/* Device bytes held by checkpoints, tape and adjoint buffers. */
+int64_t fx_adjoint_memory(const fx_system* sys);
+
 /* Last error message of the calling thread. */
 const char* fx_last_error(void);
 const char* fx_version(void);

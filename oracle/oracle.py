@@ -51,11 +51,13 @@ def lib():
             fn = getattr(L, "orc_grad_" + name)
             fn.restype = C.c_int
             fn.argtypes = [P, C.c_int, P, P, P, P, P, real, C.c_int, C.c_int, C.c_double, C.c_double, P,
-                           C.c_int, P, P, P, P, P]
+                           C.c_int, P, P, P, P, P, P]
             fn = getattr(L, "orc_loss_" + name)
             fn.restype = C.c_double
             fn.argtypes = [P, C.c_int, P, P, P, P, P, real, C.c_int, C.c_int, C.c_double, C.c_double, P,
                            C.c_int]
+        L.orc_grad_mixed.restype = C.c_int
+        L.orc_grad_mixed.argtypes = [P, C.c_int, P, P, P, P, P, C.c_float, C.c_int, P, C.c_int, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -168,21 +170,23 @@ def _coef(coef):
 def grad(params, state, nsteps, target, fp64=False, coef=None, threads=1):
     """Loss (1/n) sum |x_T - x*|^2 of an nsteps rollout from `state` (pos, vel,
     veval, clr, id, time) against target (n x 3, by particle id), and its
-    gradient with respect to the initial vel and veval (by id), viscosity and
+    gradient with respect to the initial pos, vel and veval (by id), viscosity and
     internal stiffness.  coef = (visc, k) evaluates the force with those
     values in the rollout's precision instead of the FParams_t floats."""
     dt = np.float64 if fp64 else np.float32
     n, pos, vel, veval, clr, ids = _initial(state, dt)
     tgt = np.ascontiguousarray(target, dt).reshape(-1, 3)
     assert tgt.shape[0] == n
+    gp = np.zeros((n, 3), dt)
     gv = np.zeros((n, 3), dt)
     ge = np.zeros((n, 3), dt)
     loss, gvisc, gk = C.c_double(), C.c_double(), C.c_double()
     use, visc, k = _coef(coef)
     fn = lib().orc_grad_f64 if fp64 else lib().orc_grad_f32
     fn(C.byref(params), n, _ptr(pos), _ptr(vel), _ptr(veval), _ptr(clr), _ptr(ids), float(state.get("time", 0.0)),
-       nsteps, use, visc, k, _ptr(tgt), threads, C.byref(loss), _ptr(gv), _ptr(ge), C.byref(gvisc), C.byref(gk))
-    return {"loss": loss.value, "gvel": gv, "gveval": ge, "gvisc": gvisc.value, "gk": gk.value}
+       nsteps, use, visc, k, _ptr(tgt), threads, C.byref(loss), _ptr(gp), _ptr(gv), _ptr(ge), C.byref(gvisc),
+       C.byref(gk))
+    return {"loss": loss.value, "gpos": gp, "gvel": gv, "gveval": ge, "gvisc": gvisc.value, "gk": gk.value}
 
 
 def loss(params, state, nsteps, target, fp64=True, coef=None, threads=1):
@@ -201,3 +205,16 @@ def by_id(values, ids):
     out = np.empty_like(values)
     out[np.asarray(ids, np.int64)] = values
     return out
+
+
+def grad_mixed(params, state, nsteps, target, threads=1):
+    """FP64 adjoint over the FP32 rollout's tape (same outputs as grad, float64)."""
+    n, pos, vel, veval, clr, ids = _initial(state, np.float32)
+    tgt = np.ascontiguousarray(target, np.float32).reshape(-1, 3)
+    g = {k: np.zeros((n, 3)) for k in ("gpos", "gvel", "gveval")}
+    loss, gvisc, gk = C.c_double(), C.c_double(), C.c_double()
+    lib().orc_grad_mixed(C.byref(params), n, _ptr(pos), _ptr(vel), _ptr(veval), _ptr(clr), _ptr(ids),
+                         float(state.get("time", 0.0)), nsteps, _ptr(tgt), threads, C.byref(loss), _ptr(g["gpos"]),
+                         _ptr(g["gvel"]), _ptr(g["gveval"]), C.byref(gvisc), C.byref(gk))
+    g.update({"loss": loss.value, "gvisc": gvisc.value, "gk": gk.value})
+    return g

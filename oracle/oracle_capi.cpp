@@ -236,19 +236,20 @@ void load_initial(oracle::State<R>& S, const FParams_t* P, int n, const R* pos, 
 extern "C" {
 
 // Loss of an nsteps rollout and its adjoint: gradients of the loss with
-// respect to the initial vel and veval of every particle (by id, 3n each),
+// respect to the initial pos, vel and veval of every particle (by id, 3n each),
 // viscosity (through vterm = lapkern visc) and internal stiffness.  With
 // use_coef the force uses visc and k in the state's precision, otherwise
 // the FParams_t values.
 #define ORC_GRAD(NAME, R)                                                                                 \
   int NAME(const FParams_t* P, int n, const R* pos, const R* vel, const R* veval, const uint32_t* clr,  \
            const uint32_t* id, R time, int nsteps, int use_coef, double visc, double k, const R* target,   \
-           int threads, double* loss, R* gvel, R* gveval, double* gvisc, double* gk) {                   \
+           int threads, double* loss, R* gpos, R* gvel, R* gveval, double* gvisc, double* gk) {          \
     oracle::State<R> S;                                                                                   \
     load_initial(S, P, n, pos, vel, veval, clr, id, time, use_coef, visc, k);                             \
     oracle::Adjoint<R> G;                                                                                 \
     *loss = oracle::loss_and_grad(*P, S, nsteps, target, G, threads);                                     \
     for (int i = 0; i < n; i++) {                                                                         \
+      gpos[3 * i] = G.gx[i].x; gpos[3 * i + 1] = G.gx[i].y; gpos[3 * i + 2] = G.gx[i].z;                  \
       gvel[3 * i] = G.gv[i].x; gvel[3 * i + 1] = G.gv[i].y; gvel[3 * i + 2] = G.gv[i].z;                  \
       gveval[3 * i] = G.ge[i].x; gveval[3 * i + 1] = G.ge[i].y; gveval[3 * i + 2] = G.ge[i].z;            \
     }                                                                                                     \
@@ -258,6 +259,25 @@ extern "C" {
   }
 ORC_GRAD(orc_grad_f32, float)
 ORC_GRAD(orc_grad_f64, double)
+
+// FP32 rollout, FP64 adjoint over its (promoted) tape; outputs in double.
+int orc_grad_mixed(const FParams_t* P, int n, const float* pos, const float* vel, const float* veval,
+                   const uint32_t* clr, const uint32_t* id, float time, int nsteps, const float* target,
+                   int threads, double* loss, double* gpos, double* gvel, double* gveval, double* gvisc,
+                   double* gk) {
+  oracle::State<float> S;
+  load_initial(S, P, n, pos, vel, veval, clr, id, time, 0, 0.0, 0.0);
+  oracle::Adjoint<double> G;
+  *loss = oracle::loss_and_grad_mixed(*P, S, nsteps, target, G, threads);
+  for (int i = 0; i < n; i++) {
+    gpos[3 * i] = G.gx[i].x; gpos[3 * i + 1] = G.gx[i].y; gpos[3 * i + 2] = G.gx[i].z;
+    gvel[3 * i] = G.gv[i].x; gvel[3 * i + 1] = G.gv[i].y; gvel[3 * i + 2] = G.gv[i].z;
+    gveval[3 * i] = G.ge[i].x; gveval[3 * i + 1] = G.ge[i].y; gveval[3 * i + 2] = G.ge[i].z;
+  }
+  *gvisc = G.gvisc;
+  *gk = G.gk;
+  return 0;
+}
 
 #define ORC_LOSS(NAME, R)                                                                                 \
   double NAME(const FParams_t* P, int n, const R* pos, const R* vel, const R* veval, const uint32_t* clr, \

@@ -225,7 +225,7 @@ void advance_bwd_one(const FParams_t& P, const TapeStep<R>& T, int s, R dt, R ss
   if (w3) {
     px -= gax * (wall * stiff * ss);
     if (P.bound_wall_force != 0.0f) {
-      const R dwall = (R)(std::cos((double)sarg) * (0.5 / 400.0) * (double)P.bound_wall_force);
+      const R dwall = std::cos(sarg) * (R(0.5) / R(400)) * (R)P.bound_wall_force;
       pz += gax * (stiff * dwall * (diff3 + wall * ss));
     }
     ex -= damp * gax;
@@ -403,6 +403,52 @@ double loss_and_grad(const FParams_t& P, State<R>& S, int nsteps, const R* targe
   rollout(P, S, nsteps, &tape, threads);
   const double L = loss_seed(S, target, G);
   for (int t = nsteps - 1; t >= 0; t--) backward_step(P, K, tape[t], G, threads);
+  return L;
+}
+
+// FP64 adjoint over an FP32 forward: the FP32 rollout's tape promoted to
+// double.  Separates the adjoint's own rounding from the divergence of the
+// FP32 trajectory from the FP64 one.
+template <class Src, class Dst>
+void promote(const TapeStep<Src>& a, TapeStep<Dst>& b) {
+  auto v3 = [](const std::vector<V3<Src>>& x, std::vector<V3<Dst>>& y) {
+    y.resize(x.size());
+    for (size_t i = 0; i < x.size(); i++) y[i] = V3<Dst>{(Dst)x[i].x, (Dst)x[i].y, (Dst)x[i].z};
+  };
+  v3(a.pos, b.pos);
+  v3(a.vel, b.vel);
+  v3(a.veval, b.veval);
+  v3(a.force, b.force);
+  b.press.assign(a.press.begin(), a.press.end());
+  b.id = a.id;
+  b.gcell = a.gcell;
+  b.cnt = a.cnt;
+  b.off = a.off;
+  b.time = (Dst)a.time;
+}
+
+inline double loss_and_grad_mixed(const FParams_t& P, State<float>& S, int nsteps, const float* target,
+                                  Adjoint<double>& G, int threads) {
+  std::vector<TapeStep<float>> tape;
+  rollout(P, S, nsteps, &tape, threads);
+  State<double> F;
+  F.n = S.n;
+  F.G = S.G;
+  F.pos.resize(S.n);
+  for (int i = 0; i < S.n; i++) F.pos[i] = V3<double>{S.pos[i].x, S.pos[i].y, S.pos[i].z};
+  F.id = S.id;
+  std::vector<double> tgt(3 * (size_t)S.n);
+  for (size_t i = 0; i < tgt.size(); i++) tgt[i] = target[i];
+  const double L = loss_seed(F, tgt.data(), G);
+  Coef<double> K;
+  K.vterm = (double)P.vterm;
+  K.stiff = (double)P.pintstiff;
+  K.lap = (double)P.lapkern;
+  TapeStep<double> T;
+  for (int t = nsteps - 1; t >= 0; t--) {
+    promote(tape[t], T);
+    backward_step(P, K, T, G, threads);
+  }
   return L;
 }
 

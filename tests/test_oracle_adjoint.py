@@ -1,8 +1,8 @@
 """Pins of the CPU adjoint oracle (oracle/sph_adjoint.hpp).
 
 The FP64 adjoint is checked against FP64 central differences of the FP64
-rollout (oracle.loss) in every input kind it differentiates: initial vel and
-veval components, viscosity, internal stiffness.  The rollout is piecewise
+rollout (oracle.loss) in every input kind it differentiates: initial pos, vel
+and veval components, viscosity, internal stiffness.  The rollout is piecewise
 smooth (neighbour cut-offs, wall bands and limiters switch), so a difference
 quotient whose step straddles a switch is detected by comparing two step
 sizes and that component is replaced by another seeded one.  A ballistic
@@ -80,17 +80,19 @@ def test_adjoint_velocity_components_match_central_differences(problem):
         if checked == 8:
             break
         i, a = divmod(int(c), 3)
-        for which in ("vel", "veval"):
+        for which in ("vel", "veval", "pos"):
             def f(x, which=which):
                 arr = np.zeros((n, 3))
                 arr[i, a] = x
+                if which == "pos":
+                    return _loss(P, st, target, pos=st["pos"] + arr)
                 other = np.zeros((n, 3))
                 return _loss(P, st, target, **{which: arr, ("veval" if which == "vel" else "vel"): other})
             cd = _smooth_cd(f, 0.0, 1e-6)
             if cd is None:
                 break
-            ad = g["gvel" if which == "vel" else "gveval"][i, a]
-            errs.append(abs(ad - cd) / max(abs(cd), 1e-3 * np.abs(g["gvel"]).max()))
+            ad = g["g" + which][i, a]
+            errs.append(abs(ad - cd) / max(abs(cd), 1e-3 * np.abs(g["g" + which]).max()))
         else:
             checked += 1
     assert checked == 8
