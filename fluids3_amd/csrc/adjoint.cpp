@@ -10,7 +10,8 @@
 // the recomputed steps are bit-identical to the forward's), recording each
 // step right after its force stage; then the adjoint kernels run over the
 // recorded steps in reverse.  Memory: (segments + 1) x 44 B x N of
-// checkpoints plus every x (60 B x N + 8 B x cells) of tape.
+// checkpoints plus every x (60 B x N + 8 B x cells + FX_NBR_WORDS x 4 B x N
+// of neighbour bitmask) of tape.
 #include <cstring>
 #include <vector>
 
@@ -21,7 +22,8 @@ namespace {
 
 enum { ADJ_LOSS, ADJ_ADVANCE, ADJ_FORCE, ADJ_DENSITY, ADJ_REDUCE };
 const char* kAdjNames[5] = {"adjLoss", "adjAdvance", "adjForce", "adjDensity", "adjReduce"};
-const int kAdjThreads = 256;
+const int kAdjThreads = 64;   // ADJ_THREADS: one wave per block
+const int kRedThreads = 256;  // RED_THREADS
 
 // kernel argument block (AdjArgs in adjoint_gfx950.hip)
 struct AdjArgs {
@@ -46,12 +48,18 @@ struct AdjArgs {
   void* ges;
   void* grho;
   void* partial;
+  void* arec;
+  void* drec;
+  const void* bits;
+  int nbits;
   float time;
   int accumulate;
 };
 
 const size_t kCkptBytes = 44;   // pos, vel, veval (12 B each), clr, id (4 B each)
 const size_t kTapeBytes = 60;   // pos, vel, veval, force (12 B each), rho, id, gcell (4 B each)
+// plus, per recorded step, the density kernel's neighbour bitmask
+// (FX_NBR_WORDS words per particle), written there directly by the forward
 
 }  // namespace
 
@@ -59,6 +67,8 @@ void Particles::AdjRelease() {
   for (void* p : m_AdjAlloc) (void)hipFree(p);
   m_AdjAlloc.clear();
   m_AdjCkpt = m_AdjTape = nullptr;
+  m_AdjBits = nullptr;
+  m_AdjArec = m_AdjDrec = m_AdjTabs = nullptr;
   m_AdjVec = nullptr;
   m_AdjPartial = m_AdjAccum = nullptr;
   m_AdjParams = m_AdjPBase = m_AdjBTab = m_AdjBFirst = nullptr;
@@ -104,6 +114,22 @@ int Particles::AdjConfigure(const char* path, int nsteps, int every) {
   m_AdjTape = static_cast<char*>(p);
   FX_TRY(alloc(sizeof(float) * (size_t)n * 22, &p));
   m_AdjVec = static_cast<float*>(p);
+  FX_TRY(alloc(sizeof(uint32_t) * FX_NBR_WORDS * (size_t)n * (size_t)tape_steps, &p));
+  m_AdjBits = static_cast<char*>(p);
+  FX_TRY(alloc(48 * (size_t)n, &m_AdjArec));
+  FX_TRY(alloc(16 * (size_t)n, &m_AdjDrec));
+  FX_TRY(alloc(sizeof(cuDataX) * (size_t)(tape_steps + 1), &m_AdjTabs));
+  {
+    // FAccel tables: [0] the system's own, [1 + k] with ANBRBITS in tape slot k
+    std::vector<cuDataX> tabs(tape_steps + 1);
+    m_Accel.FillTable(&tabs[0]);
+    for (int k = 0; k < tape_steps; k++) {
+      tabs[k + 1] = tabs[0];
+      tabs[k + 1].mbuf[ANBRBITS] =
+          (uint64_t)(uintptr_t)(m_AdjBits + sizeof(uint32_t) * FX_NBR_WORDS * (size_t)n * (size_t)k);
+    }
+    FX_HIP(hipMemcpyHtoD(m_AdjTabs, tabs.data(), sizeof(cuDataX) * tabs.size()), "adjoint");
+  }
   FX_TRY(alloc(sizeof(double) * 3 * (size_t)(blocks > 0 ? blocks : 1), &p));
   m_AdjPartial = static_cast<double*>(p);
   FX_TRY(alloc(sizeof(double) * 3 * (size_t)scenes, &p));
@@ -202,9 +228,15 @@ int Particles::AdjStep(int record_slot) {
   FX_TRY(InsertParticles());
   FX_TRY(PrefixScanParticles());
   FX_TRY(CountingSort());
+  char* tabs = static_cast<char*>(m_AdjTabs);
+  if (record_slot >= 0)  // the density kernel writes this step's bitmask into the tape
+    FX_HIP(hipMemcpyAsync(m_Accel.SymbolAddress(), tabs + sizeof(cuDataX) * (size_t)(record_slot + 1),
+                          sizeof(cuDataX), hipMemcpyDeviceToDevice, m_Stream), "tape");
   FX_TRY(ComputePressure());
   FX_TRY(ComputeForce());
   if (record_slot >= 0) {
+    FX_HIP(hipMemcpyAsync(m_Accel.SymbolAddress(), tabs, sizeof(cuDataX), hipMemcpyDeviceToDevice, m_Stream),
+           "tape");
     const size_t n = (size_t)m_AdjN;
     char* t = m_AdjTape + (kTapeBytes * n + 8 * (size_t)m_AdjCells) * (size_t)record_slot;
     const int ids[7] = {FPOS, FVEL, FVEVAL, FFORCE, FPRESS, FID, FGCELL};
@@ -256,7 +288,7 @@ int Particles::AdjForward(double* loss, double* ms) {
   FX_TRY(AdjLaunch(ADJ_LOSS, &A));
   int k0 = 0, k1 = 1, add = 0;
   void* rargs[] = {&m_AdjPartial, &m_AdjBFirst, &m_AdjAccum, &k0, &k1, &add};
-  FX_HIP(hipModuleLaunchKernel(m_AdjFunc[ADJ_REDUCE], (unsigned)m_AdjScenes, 1, 1, kAdjThreads, 1, 1, 0, m_Stream,
+  FX_HIP(hipModuleLaunchKernel(m_AdjFunc[ADJ_REDUCE], (unsigned)m_AdjScenes, 1, 1, kRedThreads, 1, 1, 0, m_Stream,
                                rargs, nullptr),
          "adjReduce");
   if (ms) FX_HIP(hipEventRecord(e1, m_Stream), "adjoint");
@@ -326,6 +358,10 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
       A.ges = V + 15 * n;
       A.grho = V + 18 * n;
       A.partial = m_AdjPartial;
+      A.arec = m_AdjArec;
+      A.drec = m_AdjDrec;
+      A.bits = m_AdjBits + sizeof(uint32_t) * FX_NBR_WORDS * n * (size_t)k;
+      A.nbits = (int)n;
       A.time = m_AdjTapeTime[k];
       A.accumulate = first ? 0 : 1;
       FX_TRY(AdjLaunch(ADJ_ADVANCE, &A));
@@ -342,7 +378,7 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
   if (!first) {
     int k0 = 1, k1 = 3, add = 0;
     void* rargs[] = {&m_AdjPartial, &m_AdjBFirst, &m_AdjAccum, &k0, &k1, &add};
-    FX_HIP(hipModuleLaunchKernel(m_AdjFunc[ADJ_REDUCE], (unsigned)m_AdjScenes, 1, 1, kAdjThreads, 1, 1, 0,
+    FX_HIP(hipModuleLaunchKernel(m_AdjFunc[ADJ_REDUCE], (unsigned)m_AdjScenes, 1, 1, kRedThreads, 1, 1, 0,
                                  m_Stream, rargs, nullptr),
            "adjReduce");
   }

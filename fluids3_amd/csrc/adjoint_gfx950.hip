@@ -35,7 +35,8 @@
 
 typedef unsigned int uint;
 
-#define ADJ_THREADS 256
+#define ADJ_THREADS 64   // one wave per block
+#define RED_THREADS 256
 
 template <class T>
 __device__ __forceinline__ T* gptr(T* p) {
@@ -76,6 +77,10 @@ struct AdjArgs {
   fx_float3* ges;
   float* grho;
   double* partial;  // 3 doubles per launched block: loss, dL/dk, dL/dvisc
+  float4* arec;     // per slot: (pos, rho), (veval, 0), (gF, 0), written by adjAdvance
+  float4* drec;     // per slot: (pos, grho), written by adjForce
+  const uint* bits; // the recorded step's neighbour bitmask, word w of slot i at [w * nbits + i]
+  int nbits;
   float time;
   int accumulate;  // adjForce: add to the block's partials instead of overwriting
 };
@@ -103,14 +108,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Block sum of one double per thread in a fixed order (wave butterflies, then
 // waves in index order); thread 0 gets the result.
+template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* lds) {
   v = wave_sum(v);
+  if (NT == 64) return v;
   const int w = (int)threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) lds[w] = v;
   __syncthreads();
   double s = 0.0;
   if (threadIdx.x == 0)
-    for (int k = 0; k < ADJ_THREADS / 64; k++) s += lds[k];
+    for (int k = 0; k < NT / 64; k++) s += lds[k];
   __syncthreads();
   return s;
 }
@@ -119,7 +126,7 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 // L_s = (1/n_s) sum |x - x*|^2 over scene s (FP64), seed gx[id] = (2/n_s)(x - x*),
 // gv = ge = 0.  pos/id: the final state in its slot order.
 extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjLoss(AdjArgs A) {
-  __shared__ double lds[ADJ_THREADS / 64];
+  __shared__ double lds[1];
   const Slot S = slot_of(A);
   double l = 0.0;
   if (S.i < S.p1) {
@@ -132,7 +139,7 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjLoss(AdjArgs A) {
     A.gv[id] = fx_float3{0.0f, 0.0f, 0.0f};
     A.ge[id] = fx_float3{0.0f, 0.0f, 0.0f};
   }
-  l = block_sum(l, lds);
+  l = block_sum<ADJ_THREADS>(l, lds);
   if (threadIdx.x == 0) A.partial[3 * blockIdx.x] = l;
 }
 
@@ -237,6 +244,9 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) 
   }
   A.gv[id] = fx_float3{gvx, gvy, gvz};
   A.gF[s] = fx_float3{gax * m, gay * m, gaz * m};
+  A.arec[3 * (size_t)s] = make_float4(pos.x, pos.y, pos.z, A.rho[s]);
+  A.arec[3 * (size_t)s + 1] = make_float4(ve.x, ve.y, ve.z, 0.0f);
+  A.arec[3 * (size_t)s + 2] = make_float4(gax * m, gay * m, gaz * m, 0.0f);
   float px = gxo.x, py = gxo.y, pz = gxo.z;
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
   if (w1) {
@@ -281,76 +291,211 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) 
   A.ges[s] = fx_float3{ex, ey, ez};
 }
 
-// ------------------------------------------------------------------ force
-extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjForce(AdjArgs A) {
-  __shared__ double lds[ADJ_THREADS / 64];
-  const Slot S = slot_of(A);
-  const FParams_t& P = *S.P;
-  float ak = 0.0f, av = 0.0f;
-  const int i = S.i;
-  if (i < S.p1 && A.gcell[i] != FX_GRID_UNDEF) {
-    const int gc = A.gcell[i];
-    const int base = gc - (P.gridRes.z + 1) * P.gridRes.x - 1;
-    const int srch = P.gridSrch;
-    const float rd2 = P.rd2, d2 = P.d2, h = P.psmoothradius, rest = P.prest_dens;
-    const float stiff = P.pintstiff, vterm = P.vterm, lap = P.lapkern;
-    const float kp = (P.sim_scale * -0.5f) * P.spikykern;
-    const fx_float3 p = A.pos[i], vi = A.veval[i], gi = A.gF[i];
-    const float di = A.rho[i];
-    const float pi = (di - rest) * stiff;
-    const float idi = 1.0f / di;
-    const float qi = di - rest;
-    float gpx = 0.0f, gpy = 0.0f, gpz = 0.0f, gex = 0.0f, gey = 0.0f, gez = 0.0f, gr = 0.0f;
-    for (int r = 0; r < srch * srch; r++) {
-      const int c0 = base + P.gridAdj[r * srch];
-      const int c1 = c0 + srch - 1;
-      const int jb = (int)A.off[c0], je = (int)(A.off[c1] + A.cnt[c1]);
-      for (int j = jb; j < je; j++) {
-        const fx_float3 q = A.pos[j];
-        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-        const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        if (!(dsq < rd2 && dsq > 0.0f)) continue;
-        const float rr = sqrtf(dsq * d2);
-        const float c = h - rr;
-        const float dj = A.rho[j];
-        const float pj = (dj - rest) * stiff;
-        const float inv = 1.0f / (rr * (di * dj));
-        const float invd = inv * rr;
-        const float ir = inv * (di * dj);
-        const float kc2i = (kp * (c * c)) * inv;
-        const float pp = pi + pj;
-        const float Ac = kc2i * pp;
-        const float Bc = (vterm * c) * invd;
-        const fx_float3 gj = A.gF[j], vj = A.veval[j];
-        const float lx = gi.x - gj.x, ly = gi.y - gj.y, lz = gi.z - gj.z;
-        const float wx = vj.x - vi.x, wy = vj.y - vi.y, wz = vj.z - vi.z;
-        const float su = fmaf(dz, lz, fmaf(dy, ly, dx * lx));
-        const float sv = fmaf(wz, lz, fmaf(wy, ly, wx * lx));
-        const float Ar = -(kp * pp) * (c * (rr + rr + c)) * (invd * (ir * ir));
-        const float Br = -vterm * invd;
-        const float g = (d2 * ir) * fmaf(Ar, su, Br * sv);
-        gpx = fmaf(Ac, lx, fmaf(g, dx, gpx));
-        gpy = fmaf(Ac, ly, fmaf(g, dy, gpy));
-        gpz = fmaf(Ac, lz, fmaf(g, dz, gpz));
-        gex = fmaf(-Bc, lx, gex);
-        gey = fmaf(-Bc, ly, gey);
-        gez = fmaf(-Bc, lz, gez);
-        gr = fmaf(kc2i * stiff, su, gr) - fmaf(Ac, su, Bc * sv) * idi;
-        const float ui = fmaf(dz, gi.z, fmaf(dy, gi.y, dx * gi.x));
-        const float wi = fmaf(wz, gi.z, fmaf(wy, gi.y, wx * gi.x));
-        ak = fmaf(kc2i * (qi + (dj - rest)), ui, ak);
-        av = fmaf((lap * c) * invd, wi, av);
+// ------------------------------------------------------- neighbour rings
+//
+// The recorded forward step's density kernel left, per particle, one bit per
+// stencil candidate (set for the in-radius pairs; row r occupies a
+// word-aligned run of ceil(len/32) words at [w * N + i]).  Both pair kernels
+// below turn those bits into neighbour indices in a per-lane ring in LDS and
+// evaluate queued pairs in wave-wide batches (every lane takes as many pairs
+// as the shortest active queue holds), so the pair arithmetic runs with
+// nearly all lanes busy and no candidate is tested twice.  Each lane visits
+// its pairs in candidate order (the order of the CPU oracle's loops).  One
+// wave per block; a particle whose candidates needed more than FX_NBR_WORDS
+// words has no bits and tests its candidates in place.
+#define AQ_SLOTS 1024  // ring slots per block: 16 per lane
+
+__device__ __forceinline__ int wave_max_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ void row_of(const FParams_t& P, int base, int r, const uint* off, const uint* cnt,
+                                       int& jb, int& je) {
+  const int srch = P.gridSrch;
+  const int c0 = base + P.gridAdj[r * srch];
+  const int c1 = c0 + srch - 1;
+  jb = (int)off[c0];
+  je = (int)(off[c1] + cnt[c1]);
+}
+
+__device__ __forceinline__ int words_of(const FParams_t& P, int base, const uint* off, const uint* cnt) {
+  int w = 0;
+  for (int r = 0; r < P.gridSrch * P.gridSrch; r++) {
+    int jb, je;
+    row_of(P, base, r, off, cnt, jb, je);
+    w += (je - jb + 31) >> 5;
+  }
+  return w;
+}
+
+// Walks the lane's recorded pairs through the ring; op.eval(j, ok) evaluates
+// pair j and keeps the result only when ok (a false call is a padding slot:
+// j = 0, loads stay in bounds, nothing is accumulated).  All 64 lanes must
+// call it (wave-wide maxima).
+template <class Op>
+__device__ __forceinline__ void ring_walk(const FParams_t& P, const AdjArgs& A, int i, int base, bool use,
+                                          uint* fq, Op& op) {
+  const int t = threadIdx.x;
+  const int fqn = AQ_SLOTS / 64;
+  const int srch = P.gridSrch;
+  const size_t NN = (size_t)A.nbits;
+  int head = 0, qn = 0, W = 0;
+  auto drain = [&](int f) {
+#pragma unroll 2
+    for (int e = 0; e < f; e++) {
+      const bool ok = e < qn;
+      const uint j = ok ? fq[head * 64 + t] : 0u;
+      op.eval(j, ok);
+      if (ok) head = (head + 1 == fqn) ? 0 : head + 1;
+    }
+    qn = max(qn - f, 0);
+  };
+  for (int r = 0; r < srch * srch; r++) {
+    int jb = 0, je = 0;
+    if (use) row_of(P, base, r, A.off, A.cnt, jb, je);
+    const int nw = (je - jb + 31) >> 5;
+    const int maxw = min(wave_max_i(nw), FX_NBR_WORDS);
+    for (int w = 0; w < maxw; w++) {
+      uint word = 0;
+      if (w < nw) word = A.bits[(size_t)(W + w) * NN + i];
+      for (int h = 0; h < 32; h += 16) {
+        uint m = (word >> h) & 0xffffu;
+        const int pc = __popc(m);
+        if (__any(qn + pc > fqn)) {
+          const int need = wave_max_i(qn + pc - fqn);
+          drain(min(max(need, wave_min_i(use ? qn : 0x7fffffff)), fqn));
+        }
+        int tail = head + qn;
+        if (tail >= fqn) tail -= fqn;
+        while (m != 0u) {
+          const int b = __builtin_ctz(m);
+          m &= m - 1u;
+          fq[tail * 64 + t] = (uint)(jb + (w << 5) + h + b);
+          tail = (tail + 1 == fqn) ? 0 : tail + 1;
+          qn++;
+        }
       }
     }
+    W += nw;
+  }
+  drain(min(wave_max_i(qn), fqn));
+}
+
+// ------------------------------------------------------------------ force
+
+// Pair term of the force adjoint; same expression tree as the CPU oracle.
+struct ForceAdj {
+  const FParams_t* P;
+  const float4* rec;  // (pos, rho), (veval, 0), (gF, 0) per slot
+  float px, py, pz, vx, vy, vz, gx, gy, gz, di, pi, idi, qi;
+  float kp, stiff, vterm, lap, h, rest, d2;
+  float gpx, gpy, gpz, gex, gey, gez, gr, ak, av;
+
+  __device__ __forceinline__ void eval(uint j, bool ok) {
+    const float4 a = rec[3 * (size_t)j], v = rec[3 * (size_t)j + 1], q = rec[3 * (size_t)j + 2];
+    const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+    const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float rr = sqrtf(dsq * d2);
+    const float c = h - rr;
+    const float dj = a.w;
+    const float pj = (dj - rest) * stiff;
+    const float inv = 1.0f / (rr * (di * dj));
+    const float invd = inv * rr;
+    const float ir = inv * (di * dj);
+    const float kc2i = (kp * (c * c)) * inv;
+    const float pp = pi + pj;
+    const float Ac = kc2i * pp;
+    const float Bc = (vterm * c) * invd;
+    const float lx = gx - q.x, ly = gy - q.y, lz = gz - q.z;
+    const float wx = v.x - vx, wy = v.y - vy, wz = v.z - vz;
+    const float su = fmaf(dz, lz, fmaf(dy, ly, dx * lx));
+    const float sv = fmaf(wz, lz, fmaf(wy, ly, wx * lx));
+    const float Ar = -(kp * pp) * (c * (rr + rr + c)) * (invd * (ir * ir));
+    const float Br = -vterm * invd;
+    const float g = (d2 * ir) * fmaf(Ar, su, Br * sv);
+    const float ui = fmaf(dz, gz, fmaf(dy, gy, dx * gx));
+    const float wi = fmaf(wz, gz, fmaf(wy, gy, wx * gx));
+    const float n_gpx = fmaf(Ac, lx, fmaf(g, dx, gpx));
+    const float n_gpy = fmaf(Ac, ly, fmaf(g, dy, gpy));
+    const float n_gpz = fmaf(Ac, lz, fmaf(g, dz, gpz));
+    const float n_gex = fmaf(-Bc, lx, gex);
+    const float n_gey = fmaf(-Bc, ly, gey);
+    const float n_gez = fmaf(-Bc, lz, gez);
+    const float n_gr = fmaf(kc2i * stiff, su, gr) - fmaf(Ac, su, Bc * sv) * idi;
+    const float n_ak = fmaf(kc2i * (qi + (dj - rest)), ui, ak);
+    const float n_av = fmaf((lap * c) * invd, wi, av);
+    if (ok) {
+      gpx = n_gpx; gpy = n_gpy; gpz = n_gpz;
+      gex = n_gex; gey = n_gey; gez = n_gez;
+      gr = n_gr; ak = n_ak; av = n_av;
+    }
+  }
+};
+
+extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjForce(AdjArgs A) {
+  __shared__ uint fq[AQ_SLOTS];
+  const Slot S = slot_of(A);
+  const FParams_t& P = *S.P;
+  const int i = S.i;
+  const int gc = i < S.p1 ? A.gcell[i] : FX_GRID_UNDEF;
+  const bool act = gc != FX_GRID_UNDEF;
+  const int base = gc - (P.gridRes.z + 1) * P.gridRes.x - 1;
+  ForceAdj F;
+  F.P = S.P;
+  F.rec = A.arec;
+  F.kp = (P.sim_scale * -0.5f) * P.spikykern;
+  F.stiff = P.pintstiff;
+  F.vterm = P.vterm;
+  F.lap = P.lapkern;
+  F.h = P.psmoothradius;
+  F.rest = P.prest_dens;
+  F.d2 = P.d2;
+  F.px = F.py = F.pz = F.vx = F.vy = F.vz = F.gx = F.gy = F.gz = 0.0f;
+  F.di = 1.0f;
+  F.pi = F.qi = 0.0f;
+  F.idi = 1.0f;
+  F.gpx = F.gpy = F.gpz = F.gex = F.gey = F.gez = F.gr = F.ak = F.av = 0.0f;
+  if (act) {
+    const float4 a = A.arec[3 * (size_t)i], v = A.arec[3 * (size_t)i + 1], q = A.arec[3 * (size_t)i + 2];
+    F.px = a.x; F.py = a.y; F.pz = a.z;
+    F.vx = v.x; F.vy = v.y; F.vz = v.z;
+    F.gx = q.x; F.gy = q.y; F.gz = q.z;
+    F.di = a.w;
+    F.pi = (F.di - F.rest) * F.stiff;
+    F.idi = 1.0f / F.di;
+    F.qi = F.di - F.rest;
+  }
+  const bool ovf = act && words_of(P, base, A.off, A.cnt) > FX_NBR_WORDS;
+  if (ovf) {  // no recorded bits: test the candidates in place
+    const float rd2 = P.rd2;
+    for (int r = 0; r < P.gridSrch * P.gridSrch; r++) {
+      int jb, je;
+      row_of(P, base, r, A.off, A.cnt, jb, je);
+      for (int j = jb; j < je; j++) {
+        const float4 a = A.arec[3 * (size_t)j];
+        const float dx = F.px - a.x, dy = F.py - a.y, dz = F.pz - a.z;
+        const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (dsq < rd2 && dsq > 0.0f) F.eval((uint)j, true);
+      }
+    }
+  }
+  ring_walk(P, A, i, base, act && !ovf, fq, F);
+  if (act) {
     const fx_float3 x0 = A.gxs[i], e0 = A.ges[i];
-    A.gxs[i] = fx_float3{x0.x + gpx, x0.y + gpy, x0.z + gpz};
-    A.ges[i] = fx_float3{e0.x + gex, e0.y + gey, e0.z + gez};
-    A.grho[i] = gr;
+    A.gxs[i] = fx_float3{x0.x + F.gpx, x0.y + F.gpy, x0.z + F.gpz};
+    A.ges[i] = fx_float3{e0.x + F.gex, e0.y + F.gey, e0.z + F.gez};
+    A.grho[i] = F.gr;
+    A.drec[i] = make_float4(F.px, F.py, F.pz, F.gr);
   } else if (i < S.p1) {
     A.grho[i] = 0.0f;
   }
-  const double sk = block_sum((double)ak, lds);
-  const double sv = block_sum((double)av, lds);
+  const double sk = wave_sum((double)F.ak);
+  const double sv = wave_sum((double)F.av);
   if (threadIdx.x == 0) {
     double* o = A.partial + 3 * blockIdx.x;
     o[1] = A.accumulate ? o[1] + sk : sk;
@@ -359,55 +504,78 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjForce(AdjArgs A) {
 }
 
 // ---------------------------------------------------------------- density
+
+struct DensityAdj {
+  const float4* rec;  // (pos, grho) per slot
+  float px, py, pz, gi, d2, rd2, k6;
+  float gpx, gpy, gpz;
+
+  __device__ __forceinline__ void eval(uint j, bool ok) {
+    const float4 a = rec[j];
+    const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+    const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float t = (rd2 - dsq) * d2;
+    const float cf = ((gi + a.w) * (t * t)) * k6;
+    const float nx = fmaf(cf, dx, gpx), ny = fmaf(cf, dy, gpy), nz = fmaf(cf, dz, gpz);
+    if (ok) {
+      gpx = nx; gpy = ny; gpz = nz;
+    }
+  }
+};
+
 extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjDensity(AdjArgs A) {
+  __shared__ uint fq[AQ_SLOTS];
   const Slot S = slot_of(A);
-  const int i = S.i;
-  if (i >= S.p1) return;
   const FParams_t& P = *S.P;
-  float gpx = 0.0f, gpy = 0.0f, gpz = 0.0f;
-  if (A.gcell[i] != FX_GRID_UNDEF) {
-    const int gc = A.gcell[i];
-    const int base = gc - (P.gridRes.z + 1) * P.gridRes.x - 1;
-    const int srch = P.gridSrch;
-    const float rd2 = P.rd2, d2 = P.d2;
-    const float k6 = ((-6.0f * P.pmass) * P.poly6kern) * d2;
-    const fx_float3 p = A.pos[i];
-    const float gi = A.grho[i];
-    for (int r = 0; r < srch * srch; r++) {
-      const int c0 = base + P.gridAdj[r * srch];
-      const int c1 = c0 + srch - 1;
-      const int jb = (int)A.off[c0], je = (int)(A.off[c1] + A.cnt[c1]);
+  const int i = S.i;
+  const int gc = i < S.p1 ? A.gcell[i] : FX_GRID_UNDEF;
+  const bool act = gc != FX_GRID_UNDEF;
+  const int base = gc - (P.gridRes.z + 1) * P.gridRes.x - 1;
+  DensityAdj D;
+  D.rec = A.drec;
+  D.d2 = P.d2;
+  D.rd2 = P.rd2;
+  D.k6 = ((-6.0f * P.pmass) * P.poly6kern) * P.d2;
+  D.px = D.py = D.pz = D.gi = 0.0f;
+  D.gpx = D.gpy = D.gpz = 0.0f;
+  if (act) {
+    const float4 a = A.drec[i];
+    D.px = a.x; D.py = a.y; D.pz = a.z;
+    D.gi = a.w;
+  }
+  const bool ovf = act && words_of(P, base, A.off, A.cnt) > FX_NBR_WORDS;
+  if (ovf) {
+    for (int r = 0; r < P.gridSrch * P.gridSrch; r++) {
+      int jb, je;
+      row_of(P, base, r, A.off, A.cnt, jb, je);
       for (int j = jb; j < je; j++) {
-        const fx_float3 q = A.pos[j];
-        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const float4 a = A.drec[j];
+        const float dx = D.px - a.x, dy = D.py - a.y, dz = D.pz - a.z;
         const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        if (!(dsq < rd2 && dsq > 0.0f)) continue;
-        const float t = (rd2 - dsq) * d2;
-        const float cf = ((gi + A.grho[j]) * (t * t)) * k6;
-        gpx = fmaf(cf, dx, gpx);
-        gpy = fmaf(cf, dy, gpy);
-        gpz = fmaf(cf, dz, gpz);
+        if (dsq < D.rd2 && dsq > 0.0f) D.eval((uint)j, true);
       }
     }
   }
+  ring_walk(P, A, i, base, act && !ovf, fq, D);
+  if (i >= S.p1) return;
   const uint id = A.id[i];
   const fx_float3 x0 = A.gxs[i];
-  A.gx[id] = fx_float3{x0.x + gpx, x0.y + gpy, x0.z + gpz};
+  A.gx[id] = fx_float3{x0.x + D.gpx, x0.y + D.gpy, x0.z + D.gpz};
   A.ge[id] = A.ges[i];
 }
 
 // ----------------------------------------------------------------- reduce
 // accum[3 s + k] (+)= sum of partial[3 b + k] over scene s's blocks
 // [bfirst[s], bfirst[s+1]), k in [k0, k1), in a fixed order.  One block per scene.
-extern "C" __global__ void __launch_bounds__(ADJ_THREADS)
+extern "C" __global__ void __launch_bounds__(RED_THREADS)
     adjReduce(const double* partial, const int* bfirst, double* accum, int k0, int k1, int add) {
-  __shared__ double lds[ADJ_THREADS / 64];
+  __shared__ double lds[RED_THREADS / 64];
   const int s = blockIdx.x;
   const int b0 = bfirst[s], b1 = bfirst[s + 1];
   for (int k = k0; k < k1; k++) {
     double v = 0.0;
-    for (int b = b0 + (int)threadIdx.x; b < b1; b += ADJ_THREADS) v += partial[3 * b + k];
-    v = block_sum(v, lds);
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += RED_THREADS) v += partial[3 * b + k];
+    v = block_sum<RED_THREADS>(v, lds);
     if (threadIdx.x == 0) accum[3 * s + k] = add ? accum[3 * s + k] + v : v;
   }
 }

@@ -184,6 +184,10 @@ class Particles {
   std::vector<float> m_AdjTapeTime;      // time of each recorded step of the segment
   char* m_AdjCkpt = nullptr;             // (segs+1) x 44 B x N
   char* m_AdjTape = nullptr;             // every x (60 B x N + 8 B x cells)
+  char* m_AdjBits = nullptr;             // every x FX_NBR_WORDS x 4 B x N: each recorded step's bitmask
+  void* m_AdjArec = nullptr;             // 48 B x N pair records of the force adjoint
+  void* m_AdjDrec = nullptr;             // 16 B x N pair records of the density adjoint
+  void* m_AdjTabs = nullptr;             // FAccel tables routing ANBRBITS into the tape
   float* m_AdjVec = nullptr;             // gx, gv, ge, gF, gxs, ges (3N floats each), grho (N), target (3N)
   double* m_AdjPartial = nullptr;        // 3 per block
   double* m_AdjAccum = nullptr;          // 3 per scene

@@ -14,7 +14,7 @@ import os
 import numpy as np
 import pytest
 
-from fluids3_amd import Particles, abi, derive_params, fill_points, scenes
+from fluids3_amd import Particles, abi, derive_params, fill_points, scene_spacing, scenes
 from fluids3_amd.rollout import target_positions
 from oracle import oracle
 from parity import bits_equal
@@ -52,6 +52,23 @@ def test_adjoint_bitexact_against_fp32_oracle(ps):
     for k in ("gpos", "gvel", "gveval"):
         assert bits_equal(g[k], r[k]), (k, np.abs(g[k] - r[k]).max())
     assert g["loss"][0] == pytest.approx(r["loss"], rel=1e-9)
+    assert g["gvisc"][0] == pytest.approx(r["gvisc"], rel=1e-9)
+    assert g["gk"][0] == pytest.approx(r["gk"], rel=1e-9)
+
+
+def test_adjoint_dense_block_without_bitmask(ps):
+    # ~180 particles per cell: interior particles need more than FX_NBR_WORDS
+    # bitmask words, so their pairs are found by testing candidates in place
+    sc = scenes.dam_break(6000, (-40, 0, -40), (40, 60, 40), (-4, 4, -4), (4, 12, 4))
+    T = 3
+    target = target_positions(ps, sc, T)
+    g = gpu_grad(ps, sc, T, 2, target)
+    P = derive_params(sc)
+    pos, clr = fill_points(sc)
+    assert scene_spacing(sc) < 0.5
+    r = oracle.grad(P, {"pos": pos, "clr": clr}, T, target, fp64=False, threads=THREADS)
+    for k in ("gpos", "gvel", "gveval"):
+        assert bits_equal(g[k], r[k]), (k, np.abs(g[k] - r[k]).max())
     assert g["gvisc"][0] == pytest.approx(r["gvisc"], rel=1e-9)
     assert g["gk"][0] == pytest.approx(r["gk"], rel=1e-9)
 
