@@ -67,6 +67,7 @@ def test_adjoint_scalar_parameters_match_central_differences(problem):
             if cd is not None:
                 break
         assert cd is not None, name
+        print(f"d/d{name}: adjoint {ad:.10e}, central difference {cd:.10e}, rel {abs(ad - cd) / abs(cd):.2e}")
         assert abs(ad - cd) <= 1e-4 * abs(cd), (name, ad, cd)
 
 
@@ -96,6 +97,7 @@ def test_adjoint_velocity_components_match_central_differences(problem):
         else:
             checked += 1
     assert checked == 8
+    print(f"pos/vel/veval components: max rel error {max(errs):.2e} over {len(errs)}")
     assert max(errs) <= 1e-4, errs
 
 
