@@ -68,6 +68,12 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="cfg1: steps of the forward+adjoint rollout timed beside the plain step (0 = skip)")
     ap.add_argument("--checkpoint-every", type=int, default=10)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default block sizes instead of tuning them after the warm-up")
+    ap.add_argument("--code-object", default=None, help="step code object to load (default: the built one)")
+    ap.add_argument("--stage-threads", default="",
+                    help="threads per block of the six stages, comma separated (insert,prefix,sort,pressure,force,"
+                         "advance); empty = the library's defaults")
     ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly")
     ap.add_argument("--stats-every", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -104,7 +110,10 @@ def main():
     batch = args.workload == "cfg2"
     K = args.steps if args.steps is not None else (100 if batch else 200)
     W = args.warmup
-    ps = Particles()
+    ps = Particles() if args.code_object is None else Particles(args.code_object)
+    if args.stage_threads:
+        for k, v in enumerate(args.stage_threads.split(",")):
+            ps.SetStageBlockSize(k, int(v))
     if batch:
         # 64 independent dam breaks, counts log-uniform in [8192, 262144],
         # viscosity uniform in [0.2, 0.5], each box filled to 0.4
@@ -117,11 +126,15 @@ def main():
     N, G = ps.NumPoints(), ps.CellTableSize() - ps.NumScenes()
     p = ps.params
 
-    # warm-up, then the timed region (one graph of K steps with per-stage events;
+    # warm-up, block-size tuning on the warmed-up state (the tuner restores
+    # it), then the timed region (one graph of K steps with per-stage events;
     # eager launches with the same events if the graph cannot carry them)
     use_graph = not args.no_graph
     ps.Run(W)
     ps.Synchronize()
+    tuned = None
+    if not args.no_autotune and not args.stage_threads:
+        tuned = {k: v[0] for k, v in ps.Autotune(5).items()}
     try:
         ms_total, stage_ms = ps.BenchSteps(K, use_graph=use_graph)
     except Exception as e:
@@ -133,6 +146,7 @@ def main():
         ps.Run(W)
         ps.Synchronize()
         ms_total, stage_ms = ps.BenchSteps(K, use_graph=False)
+    thr = [ps.StageBlockSize(k) for k in range(6)]
     value = N * K / (ms_total * 1e-3)
     ms_step = ms_total / K
 
@@ -245,7 +259,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded jittered lattice)",
         "config": {"workload": describe(args.workload, sc, N, G), "particles": N, "grid_cells": G,
-                   "graph": use_graph, "candidates_per_particle": round(C, 2),
+                   "graph": use_graph, "stage_threads": thr, "autotuned": tuned is not None, "candidates_per_particle": round(C, 2),
                    "pairs_per_particle": round(P, 2), "stages": stages,
                    **({"single_scene": single} if single else {})},
         "roofline": roof,

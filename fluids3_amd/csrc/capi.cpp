@@ -189,6 +189,11 @@ int fx_set_block_size(fx_system* s, int32_t threads) {
   return s->p.SetBlockSize(threads);
 }
 
+int fx_autotune(fx_system* s, int32_t nsteps, int32_t* threads_out, double* ms_out) {
+  if (s == nullptr) return fx::fail(FX_EINVAL, "null system");
+  return s->p.Autotune(nsteps, threads_out, ms_out);
+}
+
 int fx_adjoint_configure(fx_system* s, const char* path, int32_t nsteps, int32_t every) {
   if (s == nullptr) return fx::fail(FX_EINVAL, "null system");
   return s->p.AdjConfigure(path, nsteps, every);

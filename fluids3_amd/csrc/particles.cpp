@@ -276,7 +276,18 @@ int Particles::RebuildAccelGrid() {
   FX_TRY(m_Accel.AddBuffer(AAUXSCAN2, "scan2", sizeof(unsigned), numElem3, DT_CUMEM));
   FX_TRY(m_Accel.AddBuffer(ANBRBITS, "nbrbits", sizeof(unsigned), (uint64_t)FX_NBR_WORDS * mMaxPoints, DT_CUMEM));
   FX_TRY(m_Accel.AddBuffer(AFREC, "frec", 32, mMaxPoints, DT_CUMEM));
+  // out-of-range flags of the particles and their scan levels
+  const int tail1 = mMaxPoints > 0 ? mMaxPoints : 1;
+  const int tail2 = tail1 / SCAN_ELEMS + 1;
+  const int tail3 = tail2 / SCAN_ELEMS + 1;
+  FX_TRY(m_Accel.AddBuffer(ATAILFLAG, "tailflag", sizeof(unsigned), tail1, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(ATAILSCAN, "tailscan", sizeof(unsigned), tail1, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(ATAILAUX1, "tailaux1", sizeof(unsigned), tail2, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(ATAILAUXS1, "tailscan1", sizeof(unsigned), tail2, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(ATAILAUX2, "tailaux2", sizeof(unsigned), tail3, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(ATAILAUXS2, "tailscan2", sizeof(unsigned), tail3, DT_CUMEM));
   for (int b = 0; b <= AAUXSCAN2; b++) m_Accel.SetBufferUsage(b, DT_UINT);
+  for (int b = ATAILFLAG; b <= ATAILAUXS2; b++) m_Accel.SetBufferUsage(b, DT_UINT);
   FX_TRY(UpdateGPUAccess());
   return FX_OK;
 }
@@ -362,10 +373,13 @@ int Particles::InsertParticles() {
 }
 
 int Particles::PrefixScanParticles() {
-  // particles.cpp:774-815, over gridTotal+1 counts
-  return ScanLevels(m_Accel.gpu(AGRIDCNT), m_Accel.gpu(AGRIDOFF), m_Accel.gpu(AAUXARRAY1),
+  // particles.cpp:774-815, over gridTotal+1 counts; then the out-of-range
+  // flags, whose exclusive scan ranks the tail bucket in particle order
+  FX_TRY(ScanLevels(m_Accel.gpu(AGRIDCNT), m_Accel.gpu(AGRIDOFF), m_Accel.gpu(AAUXARRAY1),
                     m_Accel.gpu(AAUXSCAN1), m_Accel.gpu(AAUXARRAY2), m_Accel.gpu(AAUXSCAN2),
-                    m_ScanCells);
+                    m_ScanCells));
+  return ScanLevels(m_Accel.gpu(ATAILFLAG), m_Accel.gpu(ATAILSCAN), m_Accel.gpu(ATAILAUX1),
+                    m_Accel.gpu(ATAILAUXS1), m_Accel.gpu(ATAILAUX2), m_Accel.gpu(ATAILAUXS2), mNumPoints);
 }
 
 int Particles::ScanLevels(void* array1, void* scan1, void* array2, void* scan2, void* array3, void* scan3,

@@ -104,6 +104,7 @@ class Particles {
   int TimeSteps(int nsteps, bool use_graph, double* ms_total);
   int BenchSteps(int nsteps, bool use_graph, double* ms_total, double* stage_ms);
   int CountPairs(int64_t* cand, int64_t* pairs);
+  int Autotune(int nsteps, int* chosen, double* stage_ms);  // autotune.cpp
 
   // reverse mode (adjoint.cpp): checkpointed rollouts and their gradients
   int AdjConfigure(const char* code_object_path, int nsteps, int every);

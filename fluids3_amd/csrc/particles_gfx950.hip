@@ -176,6 +176,8 @@ __device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S) {
       gs = S.c0 + P.gridTotal;
       pbuf<int>(FPnts, FGCELL)[i] = FX_GRID_UNDEF;
     }
+    uint* tflag = pbuf<uint>(FAccel, ATAILFLAG);
+    if (tflag != nullptr) tflag[i] = in ? 0u : 1u;
   }
   uint* cnt = pbuf<uint>(FAccel, AGRIDCNT);
   unsigned long long todo = __ballot(active);
@@ -274,6 +276,8 @@ __device__ __forceinline__ void sort_index_impl(const FParams_t& P, const Span S
 // in-cell order: a particle's slot in its cell is the number of members with
 // a smaller pre-sort index (read from the member list), so the sorted order
 // equals a stable sort by cell key and does not depend on atomic timing.
+// The tail bucket of out-of-range particles, which can hold many, is ranked
+// by the exclusive scan of the insert stage's out-of-range flags instead.
 // Cells above FX_RANK_CAP members keep the atomic order.  The sort moves the
 // state that outlives a step (pos, vel, veval, clr, id) and the fresh gcell
 // and gndx; press and force are recomputed by the next two stages.
@@ -287,8 +291,12 @@ __device__ __forceinline__ void sort_full_impl(const FParams_t& P, const Span S)
   const uint* grid = pbuf<const uint>(FAccel, AGRID);
   const uint o = off[cell];
   const uint n = cnt[cell];
+  const uint* tscan = pbuf<const uint>(FAccel, ATAILSCAN);
   int rank;
-  if (n <= FX_RANK_CAP) {
+  if (c == FX_GRID_UNDEF && tscan != nullptr) {
+    // tail bucket: the number of the scene's out-of-range particles before i
+    rank = (int)(tscan[i] - tscan[S.p0]);
+  } else if (n <= FX_RANK_CAP) {
     rank = 0;
     for (uint k = 0; k < n; k++) rank += (grid[o + k] < (uint)i) ? 1 : 0;
   } else {

@@ -54,6 +54,7 @@ SIGNATURES = {
     "fx_set_block_size": (C.c_int, [_P, C.c_int32]),
     "fx_set_stage_block_size": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "fx_stage_block_size": (C.c_int32, [_P, C.c_int32]),
+    "fx_autotune": (C.c_int, [_P, C.c_int32, _P, _P]),
     "fx_adjoint_configure": (C.c_int, [_P, C.c_char_p, C.c_int32, C.c_int32]),
     "fx_adjoint_set_target": (C.c_int, [_P, _P, C.c_int64]),
     "fx_adjoint_forward": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -283,6 +284,13 @@ class Particles:
         c, p = C.c_int64(), C.c_int64()
         check(lib().fx_count_pairs(self._h, C.byref(c), C.byref(p)), "fx_count_pairs")
         return c.value, p.value
+
+    def Autotune(self, nsteps=5):
+        """Picks each stage's block size (fx_autotune); returns {stage: (threads, ms)}."""
+        th = (C.c_int32 * abi.NUM_STAGES)()
+        ms = (C.c_double * abi.NUM_STAGES)()
+        check(lib().fx_autotune(self._h, nsteps, th, ms), "fx_autotune")
+        return {abi.STAGE_NAMES[k]: (th[k], ms[k]) for k in range(abi.NUM_STAGES)}
 
     # ---- reverse mode
     def AdjointConfigure(self, nsteps, every, code_object=None):

@@ -176,6 +176,15 @@ int fx_set_block_size(fx_system* sys, int32_t threads);
 int fx_set_stage_block_size(fx_system* sys, int32_t stage, int32_t threads);
 int32_t fx_stage_block_size(const fx_system* sys, int32_t stage);
 
+/* Autotuner (no reference counterpart; the reference launches everything with
+ * 512 threads per block, particles.cpp:417-418): profiles nsteps steps from
+ * the current state for each block size in {64, 128, 256, 512, 1024}, keeps
+ * the fastest per stage, checks that a step under the chosen sizes is
+ * bit-identical to a step under the previous ones (FX_EMISMATCH otherwise),
+ * and leaves the system at the state it started from with the chosen sizes.
+ * threads_out / ms_out (optional): FX_NUM_STAGES entries each. */
+int fx_autotune(fx_system* sys, int32_t nsteps, int32_t* threads_out, double* ms_out);
+
 /* ---- reverse mode (no reference counterpart: the reference has no
  * differentiable path) ----
  *

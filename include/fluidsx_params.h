@@ -18,6 +18,9 @@
  *     results can be compared per particle.
  *   - ANBRBITS, AFREC: neighbour bitmask and packed per-particle record that
  *     the density stage leaves for the force stage.
+ *   - ATAILFLAG .. ATAILAUXS2: out-of-range flags and their exclusive scan,
+ *     which give the tail bucket a stable (particle-order) ranking however
+ *     many particles it holds.
  *   - AGRIDCNT / AGRIDOFF hold gridTotal+1 entries.  Slot gridTotal counts the
  *     particles whose cell is GRID_UNDEF, so the scan places them in a
  *     deterministic tail after every valid cell instead of leaving stale data
@@ -59,7 +62,13 @@ extern "C" {
  * host that does not allocate them gets the force stage without them) */
 #define ANBRBITS 7 /* neighbour bitmask: word w of particle i at [w * N + i] */
 #define AFREC 8    /* force record: 2 float4 per particle, (pos, rho), (veval, 0) */
-#define FX_NUM_ABUF 9
+#define ATAILFLAG 9   /* 1 for a particle whose cell is GRID_UNDEF, else 0 (N entries) */
+#define ATAILSCAN 10  /* exclusive scan of ATAILFLAG: a tail particle's stable rank */
+#define ATAILAUX1 11  /* scan levels of ATAILFLAG */
+#define ATAILAUXS1 12
+#define ATAILAUX2 13
+#define ATAILAUXS2 14
+#define FX_NUM_ABUF 15
 #define FX_NBR_WORDS 128 /* bitmask words per particle (4096 candidates) */
 
 #define FX_DMAXBUF 16 /* datax.h:37 */

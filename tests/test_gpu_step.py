@@ -221,6 +221,29 @@ def test_out_of_range_and_nan(ps):
     assert np.all(np.diff(g["id"][tail].astype(np.int64)) > 0)
 
 
+def test_large_tail_bucket_is_stable(ps):
+    """More out-of-range particles than a cell's member ranking covers: the
+    tail bucket is still ordered by particle index (flag scan), so runs are
+    reproducible and equal to the oracle, ids included."""
+    sc = scenes.cfg0()
+    ps.Restart(sc)
+    p = ps.params
+    pos = ps.state()["pos"].copy()
+    pos[::4, 0] += 1000.0  # 4,096 particles leave the grid
+    ps.Commit(abi.FPOS, pos)
+    s0 = ps.state()
+    ps.Run(2)
+    g = ps.state()
+    assert g["cnt"][-1] == 4096
+    compare_state(g, oracle.run(p, fresh_oracle(p, s0), 2))
+    ps.Restart(sc)
+    ps.Commit(abi.FPOS, pos)
+    ps.Run(2)
+    again = ps.state()
+    for k in ("pos", "vel", "veval", "id", "clr", "gcell"):
+        assert bits_equal(g[k], again[k]), k
+
+
 @pytest.mark.parametrize("n", [1, 2, 1023, 1024, 1025, 2047, 2048, 4097, 1024 * 1024 - 1, 1024 * 1024,
                                1024 * 1024 + 1, 3 * 1024 * 1024 + 7])
 def test_scan_lengths(ps, n):
@@ -273,3 +296,34 @@ def test_cfg1_full_size(ps):
     gf = ps.state()
     assert_close("force", gf["force"][m], r["force"][m])
     assert bits_equal(gf["force"][m], r["force"][m])
+
+
+def test_autotune_keeps_state_and_results(ps):
+    """The tuner picks a listed block size per stage, leaves the state it
+    started from, and a rollout under the tuned sizes is bit-identical to
+    one under the default sizes."""
+    defaults = [ps.StageBlockSize(k) for k in range(abi.NUM_STAGES)]
+    sc = scenes.cfg0()
+    try:
+        ps.Restart(sc)
+        ps.Run(3)
+        before = ps.state()
+        res = ps.Autotune(2)
+        assert all(t in (64, 128, 256, 512, 1024) for t, _ in res.values()), res
+        assert all(ms > 0 for _, ms in res.values()), res
+        after = ps.state()
+        for k in ("pos", "vel", "veval", "id", "clr"):
+            assert bits_equal(before[k], after[k]), k
+        assert before["time"] == after["time"]
+        ps.Run(4)
+        tuned = ps.state()
+        for k, t in enumerate(defaults):
+            ps.SetStageBlockSize(k, t)
+        ps.Restart(sc)
+        ps.Run(7)
+        plain = ps.state()
+        for k in ("pos", "vel", "veval", "press", "force", "id", "gcell", "cnt", "off"):
+            assert bits_equal(tuned[k], plain[k]), k
+    finally:
+        for k, t in enumerate(defaults):
+            ps.SetStageBlockSize(k, t)
