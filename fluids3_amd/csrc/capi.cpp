@@ -120,6 +120,22 @@ int fx_download_accel(fx_system* s, int32_t buf, void* host, int64_t count) {
   return s->p.DownloadAccel(buf, host, count);
 }
 
+int fx_sample_particles(fx_system* s, float* brick, const int32_t* res, const float* bmin, const float* bmax,
+                        float scalar) {
+  if (s == nullptr) return fx::fail(FX_EINVAL, "null system");
+  return s->p.SampleParticles(brick, res, bmin, bmax, scalar);
+}
+
+int fx_save_points(fx_system* s, const char* path) {
+  if (s == nullptr) return fx::fail(FX_EINVAL, "null system");
+  return s->p.SavePoints(path);
+}
+
+int fx_load_points(fx_system* s, const char* path) {
+  if (s == nullptr) return fx::fail(FX_EINVAL, "null system");
+  return s->p.LoadPoints(path);
+}
+
 int fx_stage(fx_system* s, int32_t stage) {
   FX_CHECK_SYS(s);
   return s->p.Stage(stage);

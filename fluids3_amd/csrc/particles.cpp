@@ -18,7 +18,7 @@ static const char* kKernelNames[FUNC_MAX] = {
     "advanceParticles",  "prefixSum",        "prefixFixup",     "countingSortIndex",
     "fxFillUint",        "advanceParticlesClock", "fxClockTick",     "fxCountPairs",
     "insertParticlesBatch", "countingSortIndexBatch", "countingSortFullBatch", "computePressureBatch",
-    "computeForceBatch", "advanceParticlesBatch", "fxCountPairsBatch"};
+    "computeForceBatch", "advanceParticlesBatch", "fxCountPairsBatch", "sampleParticles"};
 
 Particles::Particles() { memset(&m_Params, 0, sizeof(m_Params)); }
 
@@ -656,6 +656,31 @@ int Particles::ProfileStages(int nsteps, double* ms_out) {
   for (auto& e : ev) (void)hipEventDestroy(e);
   if (st != FX_OK) return st;
   for (int k = 0; k <= FX_NUM_STAGES; k++) ms_out[k] /= nsteps;
+  return FX_OK;
+}
+
+int Particles::SampleParticles(float* brick, const int* res, const float* bmin, const float* bmax, float scalar) {
+  if (mNumPoints == 0) return fx::fail(FX_ESTATE, "sample_particles: call Restart first");
+  if (m_Batch) return fx::fail(FX_ESTATE, "sample_particles: one scene only");
+  if (brick == nullptr || res == nullptr || bmin == nullptr || bmax == nullptr)
+    return fx::fail(FX_EINVAL, "sample_particles: null argument");
+  if (res[0] <= 0 || res[1] <= 0 || res[2] <= 0 || (int64_t)res[0] * res[1] * res[2] > ((int64_t)1 << 31))
+    return fx::fail(FX_ERANGE, "sample_particles: bad brick resolution %d x %d x %d", res[0], res[1], res[2]);
+  const size_t nv = (size_t)res[0] * res[1] * res[2];
+  void* d = nullptr;
+  FX_HIP(hipMalloc(&d, nv * sizeof(float)), "sample_particles");
+  fx_int3 r = {res[0], res[1], res[2]};
+  fx_float3 lo = {bmin[0], bmin[1], bmin[2]}, hi = {bmax[0], bmax[1], bmax[2]};
+  int n = mNumPoints;
+  void* args[] = {&d, &r, &lo, &hi, &n, &scalar};
+  const unsigned bx = 8, by = 8, bz = 4;
+  hipError_t e = hipModuleLaunchKernel(m_Func[FUNC_SAMPLE], (unsigned)((res[0] + bx - 1) / bx),
+                                       (unsigned)((res[1] + by - 1) / by), (unsigned)((res[2] + bz - 1) / bz), bx, by,
+                                       bz, 0, m_Stream, args, nullptr);
+  if (e == hipSuccess) e = hipStreamSynchronize(m_Stream);
+  if (e == hipSuccess) e = hipMemcpyDtoH(brick, d, nv * sizeof(float));
+  (void)hipFree(d);
+  if (e != hipSuccess) return fx::hip_fail(e, "sample_particles", "sampleParticles");
   return FX_OK;
 }
 

@@ -37,7 +37,8 @@
 #define FUNC_FORCE_B 16
 #define FUNC_ADVANCE_B 17
 #define FUNC_COUNT_PAIRS_B 18
-#define FUNC_MAX 19
+#define FUNC_SAMPLE 19
+#define FUNC_MAX 20
 
 #include <map>
 #include <string>
@@ -105,6 +106,9 @@ class Particles {
   int BenchSteps(int nsteps, bool use_graph, double* ms_total, double* stage_ms);
   int CountPairs(int64_t* cand, int64_t* pairs);
   int Autotune(int nsteps, int* chosen, double* stage_ms);  // autotune.cpp
+  int SavePoints(const char* path);                         // points_io.cpp
+  int SampleParticles(float* brick, const int* res, const float* bmin, const float* bmax, float scalar);
+  int LoadPoints(const char* path);
 
   // reverse mode (adjoint.cpp): checkpointed rollouts and their gradients
   int AdjConfigure(const char* code_object_path, int nsteps, int every);

@@ -750,6 +750,61 @@ extern "C" __global__ void fxCountPairsBatch(BatchArgs B, unsigned long long* ou
   count_pairs_impl(*P, S, out);
 }
 
+// ------------------------------------------------------ volume sampling
+
+// Gaussian splat of the particles onto a brick of res.x * res.y * res.z
+// voxels spanning [bmin, bmax] (sampleParticles,
+// fluids4.0/fluid_kernels.cu:259-300), a consumer of the cell table: voxel
+// (x, y, z) sits at bmin + (x/res.x, y/res.y, z/res.z) * (bmax - bmin) and
+// sums gausskern * exp(-r^2 / h2), h2 = 2 r2 / 8, over the particles of its
+// 27 stencil cells with 0 < d^2 < rd2 (r in simulation units), in the
+// stencil's cell order; the result, times `scalar`, goes to
+// brick[(z * res.y + y) * res.x + x].  Voxels whose cell is outside
+// [1, gridRes - gridSrch] get 0 at that same index (the reference writes
+// those at (y * res.z + z) * res.x + x, a transposed index).  Positions are
+// the current set's; the cell table is the one of the last sort.
+extern "C" __global__ void sampleParticles(float* brick, fx_int3 res, fx_float3 bmin, fx_float3 bmax, int numPnts,
+                                           float scalar) {
+  const FParams_t& P = *cparams(&FParams);
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z * blockDim.z + threadIdx.z;
+  if (x >= res.x || y >= res.y || z >= res.z) return;
+  const size_t out = ((size_t)z * res.y + y) * res.x + x;
+  const float px = bmin.x + ((float)x / (float)res.x) * (bmax.x - bmin.x);
+  const float py = bmin.y + ((float)y / (float)res.y) * (bmax.y - bmin.y);
+  const float pz = bmin.z + ((float)z / (float)res.z) * (bmax.z - bmin.z);
+  const int gx = (int)((px - P.gridMin.x) * P.gridDelta.x);
+  const int gy = (int)((py - P.gridMin.y) * P.gridDelta.y);
+  const int gz = (int)((pz - P.gridMin.z) * P.gridDelta.z);
+  if (gx < 1 || gx > P.gridRes.x - P.gridSrch || gy < 1 || gy > P.gridRes.y - P.gridSrch || gz < 1 ||
+      gz > P.gridRes.z - P.gridSrch) {
+    brick[out] = 0.0f;
+    return;
+  }
+  const int nadj = (P.gridRes.z + 1) * P.gridRes.x + 1;
+  const int gs = (gy * P.gridRes.z + gz) * P.gridRes.x + gx - nadj;
+  const fx_float3* __restrict__ pos = pbuf<const fx_float3>(FPnts, FPOS);
+  const uint* __restrict__ off = pbuf<const uint>(FAccel, AGRIDOFF);
+  const uint* __restrict__ cnt = pbuf<const uint>(FAccel, AGRIDCNT);
+  const float h2 = 2.0f * P.r2 / 8.0f;
+  float v = 0.0f;
+  for (int c = 0; c < P.gridAdjCnt; c++) {
+    const int cell = gs + P.gridAdj[c];
+    const int j0 = (int)off[cell], j1 = j0 + (int)cnt[cell];
+    for (int j = j0; j < j1 && j < numPnts; j++) {
+      const fx_float3 q = pos[j];
+      const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
+      const float dsq = dx * dx + dy * dy + dz * dz;
+      if (dsq < P.rd2 && dsq > 0.0f) {
+        const float r = sqrtf(dsq * P.d2);
+        v += P.gausskern * expf(-(r * r) / h2);
+      }
+    }
+  }
+  brick[out] = v * scalar;
+}
+
 // ----------------------------------------------------------------- advance
 
 // Integrator (particles.cu:237-322): penalty walls on bound_min/bound_max

@@ -41,6 +41,9 @@ SIGNATURES = {
     "fx_upload": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "fx_download": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
     "fx_download_accel": (C.c_int, [_P, C.c_int32, _P, C.c_int64]),
+    "fx_sample_particles": (C.c_int, [_P, _P, _P, _P, _P, C.c_float]),
+    "fx_save_points": (C.c_int, [_P, C.c_char_p]),
+    "fx_load_points": (C.c_int, [_P, C.c_char_p]),
     "fx_stage": (C.c_int, [_P, C.c_int32]),
     "fx_run": (C.c_int, [_P, C.c_int32]),
     "fx_synchronize": (C.c_int, [_P]),
@@ -247,6 +250,25 @@ class Particles:
         if comp > 1 and a.shape[1:] != (comp,):
             raise ValueError(f"buffer {buf} expects shape (n, {comp})")
         check(lib().fx_upload(self._h, buf, a.ctypes.data, n), f"upload {buf}")
+
+    def SampleParticles(self, res, bmin, bmax, scalar=1.0):
+        """Gaussian splat of the particles on a brick (sampleParticles); returns a
+        float32 array of shape (res_z, res_y, res_x)."""
+        r = np.ascontiguousarray(res, np.int32)
+        lo = np.ascontiguousarray(bmin, np.float32)
+        hi = np.ascontiguousarray(bmax, np.float32)
+        out = np.empty((int(r[2]), int(r[1]), int(r[0])), np.float32)
+        check(lib().fx_sample_particles(self._h, out.ctypes.data, r.ctypes.data, lo.ctypes.data, hi.ctypes.data,
+                                        float(scalar)), "fx_sample_particles")
+        return out
+
+    def SavePoints(self, path):
+        """Writes the particle state as a .pts file (Fluids 4.0 layout plus veval and id)."""
+        check(lib().fx_save_points(self._h, os.fsencode(path)), "fx_save_points")
+
+    def LoadPoints(self, path):
+        """Reads a .pts file written by SavePoints or by Fluids 4.0 (same particle count)."""
+        check(lib().fx_load_points(self._h, os.fsencode(path)), "fx_load_points")
 
     def RetrieveAccel(self, buf, n):
         out = np.empty(n, dtype=np.uint32)

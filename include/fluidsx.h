@@ -130,6 +130,16 @@ int fx_download(fx_system* sys, int32_t buf, void* host, int64_t count);
 /* Grid buffers AGRID (N uints), AGRIDCNT / AGRIDOFF (gridTotal+1 uints). */
 int fx_download_accel(fx_system* sys, int32_t buf, void* host, int64_t count);
 
+/* Particle state files in the Fluids 4.0 .pts layout
+ * (FluidSystem::SavePoints / RunPlayback, fluids4.0/fluid_system.cpp:1511-1545,
+ * 1787-1843): int32 count, int32 fields, then per field int32 type
+ * (0 char, 1 int, 2 float), int32 channels and the values.  Written: pos,
+ * vel, clr (the 4.0 fields), then veval and id.  Loading needs the same
+ * particle count as the system and accepts 4.0 files (veval = vel,
+ * ids 0..n-1 then). */
+int fx_save_points(fx_system* sys, const char* path);
+int fx_load_points(fx_system* sys, const char* path);
+
 /* ---- stepping ---- */
 
 /* One per-function stage (FX_STAGE_*), enqueued on the stream. */
@@ -169,6 +179,15 @@ int fx_bench_steps(fx_system* sys, int32_t nsteps, int32_t use_graph, double* ms
  * candidates and of in-radius pairs (0 < d^2 < rd2); the role of the
  * reference's neighbour-count validation (fluids4.0/fluid_system.cpp:959-1011). */
 int fx_count_pairs(fx_system* sys, int64_t* candidates, int64_t* pairs);
+
+/* Gaussian splat of the particles onto a res[0] x res[1] x res[2] brick over
+ * [bmin, bmax] using the cell table (sampleParticles,
+ * fluids4.0/fluid_kernels.cu:259-300): brick[(z*res[1] + y)*res[0] + x] =
+ * scalar * sum of gausskern exp(-r^2 / (2 r2 / 8)) over the stencil's
+ * particles with 0 < d^2 < rd2.  brick: host array of the voxel count.
+ * One scene only. */
+int fx_sample_particles(fx_system* sys, float* brick, const int32_t* res, const float* bmin, const float* bmax,
+                        float scalar);
 
 /* Launch geometry: threads per block of the per-particle kernels (64..1024,
  * multiple of 64), for every stage or for one.  Results do not depend on it. */

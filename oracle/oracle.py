@@ -56,6 +56,8 @@ def lib():
             fn.restype = C.c_double
             fn.argtypes = [P, C.c_int, P, P, P, P, P, real, C.c_int, C.c_int, C.c_double, C.c_double, P,
                            C.c_int]
+        L.orc_sample.restype = C.c_int
+        L.orc_sample.argtypes = [P, C.c_int, P, P, P, P, P, P, C.c_float, P]
         L.orc_grad_mixed.restype = C.c_int
         L.orc_grad_mixed.argtypes = [P, C.c_int, P, P, P, P, P, C.c_float, C.c_int, P, C.c_int, P, P, P, P, P, P]
         _lib = L
@@ -138,9 +140,10 @@ def pairs_grid(params, state):
     cand = np.zeros(n, np.int32)
     pairs = np.zeros(n, np.int32)
     pos = np.ascontiguousarray(state["pos"], np.float32)
-    lib().orc_pairs_grid(C.byref(params), n, _ptr(pos), _ptr(np.ascontiguousarray(state["gcell"], np.int32)),
-                         _ptr(np.ascontiguousarray(state["cnt"], np.uint32)),
-                         _ptr(np.ascontiguousarray(state["off"], np.uint32)), _ptr(cand), _ptr(pairs))
+    gcell = np.ascontiguousarray(state["gcell"], np.int32)
+    cnt = np.ascontiguousarray(state["cnt"], np.uint32)
+    off = np.ascontiguousarray(state["off"], np.uint32)
+    lib().orc_pairs_grid(C.byref(params), n, _ptr(pos), _ptr(gcell), _ptr(cnt), _ptr(off), _ptr(cand), _ptr(pairs))
     return cand, pairs
 
 
@@ -218,3 +221,18 @@ def grad_mixed(params, state, nsteps, target, threads=1):
                          _ptr(g["gvel"]), _ptr(g["gveval"]), C.byref(gvisc), C.byref(gk))
     g.update({"loss": loss.value, "gvisc": gvisc.value, "gk": gk.value})
     return g
+
+
+def sample(params, state, res, bmin, bmax, scalar=1.0):
+    """Oracle sampleParticles on a cell-sorted state: (res_z, res_y, res_x) float32."""
+    # every array passed by address is bound to a name for the duration of the call
+    pos = np.ascontiguousarray(state["pos"], np.float32)
+    cnt = np.ascontiguousarray(state["cnt"], np.uint32)
+    off = np.ascontiguousarray(state["off"], np.uint32)
+    r = np.ascontiguousarray(res, np.int32)
+    lo = np.ascontiguousarray(bmin, np.float32)
+    hi = np.ascontiguousarray(bmax, np.float32)
+    out = np.zeros((int(r[2]), int(r[1]), int(r[0])), np.float32)
+    lib().orc_sample(C.byref(params), pos.shape[0], _ptr(pos), _ptr(cnt), _ptr(off), _ptr(r), _ptr(lo), _ptr(hi),
+                     float(scalar), _ptr(out))
+    return out

@@ -17,6 +17,7 @@
 //   orc_run (advance)     integrator                pinned: free fall, each wall band,
 //                                                     both limiters, GRID_UNDEF park
 //   orc_pairs_*           neighbour counts          pinned: brute force
+//   orc_sample            volume splat              pinned: single-particle closed form
 //   orc_grad_*            adjoint of a rollout      pinned (FP64): central differences of
 //                                                     orc_loss_f64 in every input kind,
 //                                                     a two-particle closed form
@@ -276,6 +277,44 @@ int orc_grad_mixed(const FParams_t* P, int n, const float* pos, const float* vel
   }
   *gvisc = G.gvisc;
   *gk = G.gk;
+  return 0;
+}
+
+// sampleParticles (fluids4.0/fluid_kernels.cu:259-300) on a cell-sorted
+// state: brick[(z*ry + y)*rx + x], voxels outside the scanned cell range 0.
+int orc_sample(const FParams_t* P, int n, const float* pos, const uint32_t* cnt, const uint32_t* off,
+               const int32_t* res, const float* bmin, const float* bmax, float scalar, float* brick) {
+  const float h2 = 2.0f * P->r2 / 8.0f;
+  const int nadj = (P->gridRes.z + 1) * P->gridRes.x + 1;
+  for (int z = 0; z < res[2]; z++)
+    for (int y = 0; y < res[1]; y++)
+      for (int x = 0; x < res[0]; x++) {
+        const float px = bmin[0] + ((float)x / (float)res[0]) * (bmax[0] - bmin[0]);
+        const float py = bmin[1] + ((float)y / (float)res[1]) * (bmax[1] - bmin[1]);
+        const float pz = bmin[2] + ((float)z / (float)res[2]) * (bmax[2] - bmin[2]);
+        const int gx = (int)((px - P->gridMin.x) * P->gridDelta.x);
+        const int gy = (int)((py - P->gridMin.y) * P->gridDelta.y);
+        const int gz = (int)((pz - P->gridMin.z) * P->gridDelta.z);
+        float v = 0.0f;
+        const bool in = !(gx < 1 || gx > P->gridRes.x - P->gridSrch || gy < 1 || gy > P->gridRes.y - P->gridSrch ||
+                          gz < 1 || gz > P->gridRes.z - P->gridSrch);
+        if (in) {
+          const int gs = (gy * P->gridRes.z + gz) * P->gridRes.x + gx - nadj;
+          for (int c = 0; c < P->gridAdjCnt; c++) {
+            const int cell = gs + P->gridAdj[c];
+            for (int j = (int)off[cell]; j < (int)(off[cell] + cnt[cell]) && j < n; j++) {
+              const float dx = px - pos[3 * j], dy = py - pos[3 * j + 1], dz = pz - pos[3 * j + 2];
+              const float dsq = dx * dx + dy * dy + dz * dz;
+              if (dsq < P->rd2 && dsq > 0.0f) {
+                const float r = std::sqrt(dsq * P->d2);
+                v += P->gausskern * std::exp(-(r * r) / h2);
+              }
+            }
+          }
+          v = v * scalar;
+        }
+        brick[((size_t)z * res[1] + y) * res[0] + x] = v;
+      }
   return 0;
 }
 

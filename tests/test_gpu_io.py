@@ -1,0 +1,126 @@
+"""GPU: .pts state files (fx_save_points / fx_load_points) in the Fluids 4.0
+layout, read back independently with numpy, loaded into a fresh system, and
+continued bit-identically."""
+import numpy as np
+import pytest
+
+from fluids3_amd import Particles, abi, scenes
+from fluids3_amd.particles import FluidsError
+from parity import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def read_pts(path):
+    """Independent reader of the layout: int32 count, int32 fields, then
+    (int32 type, int32 channels, values) per field."""
+    raw = open(path, "rb").read()
+    n, nf = np.frombuffer(raw, np.int32, 2, 0)
+    at = 8
+    fields = []
+    for _ in range(nf):
+        ft, fc = np.frombuffer(raw, np.int32, 2, at)
+        at += 8
+        dt = {0: np.uint8, 1: np.int32, 2: np.float32, 3: np.float64}[int(ft)]
+        cnt = int(n) * int(fc)
+        fields.append((int(ft), int(fc), np.frombuffer(raw, dt, cnt, at).reshape(int(n), int(fc))))
+        at += cnt * np.dtype(dt).itemsize
+    assert at == len(raw)
+    return int(n), fields
+
+
+@pytest.fixture(scope="module")
+def ps():
+    p = Particles()
+    yield p
+    p.close()
+
+
+def test_save_load_round_trip(ps, tmp_path):
+    sc = scenes.cfg0()
+    ps.Restart(sc)
+    ps.Run(3)
+    st = ps.state()
+    path = str(tmp_path / "state.pts")
+    ps.SavePoints(path)
+    n, f = read_pts(path)
+    assert n == sc.num_points and [(a, b) for a, b, _ in f] == [(2, 3), (2, 3), (0, 4), (2, 3), (1, 1)]
+    assert bits_equal(f[0][2], st["pos"]) and bits_equal(f[1][2], st["vel"])
+    assert np.array_equal(f[2][2].reshape(-1).view(np.uint32), st["clr"])
+    assert bits_equal(f[3][2], st["veval"]) and np.array_equal(f[4][2][:, 0].view(np.uint32), st["id"])
+    ps.Run(2)
+    cont = ps.state()
+    other = Particles()
+    try:
+        other.Restart(sc)
+        other.LoadPoints(path)
+        other.Run(2)
+        again = other.state()
+    finally:
+        other.close()
+    for k in ("pos", "vel", "veval", "id", "clr", "press", "force"):
+        assert bits_equal(cont[k], again[k]), k
+
+
+def test_load_fluids40_file_and_count_mismatch(ps, tmp_path):
+    sc = scenes.cfg0()
+    ps.Restart(sc)
+    st = ps.state()
+    n = sc.num_points
+    rng = np.random.default_rng(4)
+    vel = rng.uniform(-0.1, 0.1, (n, 3)).astype(np.float32)
+    path = str(tmp_path / "v40.pts")
+    with open(path, "wb") as fh:  # the three fields Fluids 4.0 writes
+        fh.write(np.array([n, 3], np.int32).tobytes())
+        fh.write(np.array([2, 3], np.int32).tobytes() + st["pos"].astype(np.float32).tobytes())
+        fh.write(np.array([2, 3], np.int32).tobytes() + vel.tobytes())
+        fh.write(np.array([0, 4], np.int32).tobytes() + st["clr"].astype(np.uint32).tobytes())
+    ps.LoadPoints(path)
+    got = ps.state()
+    assert bits_equal(got["vel"], vel) and bits_equal(got["veval"], vel)
+    assert np.array_equal(got["id"], np.arange(n, dtype=np.uint32))
+    small = scenes.scaled_dam_break(1000)
+    ps.Restart(small)
+    with pytest.raises(FluidsError):
+        ps.LoadPoints(path)
+
+
+def test_sample_particles_against_oracle(ps):
+    from oracle import oracle
+    from parity import rel_err
+
+    sc = scenes.cfg0()
+    ps.Restart(sc)
+    ps.Run(3)
+    st = ps.state()
+    res, lo, hi = (24, 20, 16), (-30.0, 0.0, -30.0), (30.0, 60.0, 30.0)
+    g = ps.SampleParticles(res, lo, hi, 2.0)
+    r = oracle.sample(ps.params, st, res, lo, hi, 2.0)
+    assert g.shape == (16, 20, 24) and np.count_nonzero(r) > 100
+    assert np.array_equal(g == 0, r == 0)
+    assert rel_err(g, r) <= 1e-5  # expf on the device against the host's expf
+
+
+def test_sample_particles_single_particle_closed_form(ps):
+    # one particle: voxels within the radius hold gausskern * exp(-r^2 / (2 r2 / 8)) * scalar
+    sc = scenes.dam_break(1, (-30, 0, -30), (30, 60, 30), (0, 20, 0), (1, 21, 1))
+    ps.Restart(sc)
+    p = ps.params
+    x0 = np.array([[0.3, 20.2, -0.1]], np.float32)
+    ps.Commit(abi.FPOS, x0)
+    ps.InsertParticles()
+    ps.PrefixScanParticles()
+    ps.CountingSort()
+    res, lo, hi = (16, 16, 16), (-4.0, 16.0, -4.0), (4.0, 24.0, 4.0)
+    g = ps.SampleParticles(res, lo, hi, 1.0)
+    idx = np.indices((16, 16, 16)).astype(np.float64)  # z, y, x
+    vx = lo[0] + idx[2] / 16 * (hi[0] - lo[0])
+    vy = lo[1] + idx[1] / 16 * (hi[1] - lo[1])
+    vz = lo[2] + idx[0] / 16 * (hi[2] - lo[2])
+    dsq = (vx - x0[0, 0]) ** 2 + (vy - x0[0, 1]) ** 2 + (vz - x0[0, 2]) ** 2
+    r2 = dsq * p.d2
+    expect = np.where((dsq < p.rd2) & (dsq > 0), p.gausskern * np.exp(-r2 / (2 * p.r2 / 8)), 0.0)
+    inside = expect > 0
+    assert inside.sum() > 50
+    assert np.allclose(g[inside], expect[inside], rtol=1e-5, atol=0)
+    assert np.all(g[~inside] == 0)
