@@ -633,9 +633,11 @@ __device__ __forceinline__ void force_impl(const FParams_t& P, const Span S, int
   int head = 0, qn = 0;  // ring holds entries head .. head+qn-1 (mod fqn)
   int W = 0;
   // evaluates the ring's first f entries of every lane (fewer if it holds
-  // fewer); the loads are unconditional (record 0 stands in for an empty
-  // slot) so that the loop is branch-free and the next entry's loads can be
-  // issued ahead of the current entry's arithmetic
+  // fewer); a lane whose ring is empty evaluates record 0 and discards the
+  // result.  The pair gathers of this loop are what bounds the kernel: the
+  // vector-memory return path is busy nearly all the time, and neither
+  // deeper rings (fewer waves fit) nor loading several entries' records
+  // ahead of their evaluation (more idle lanes) made it faster.
   auto drain = [&](int f) {
 #pragma unroll 2
     for (int e = 0; e < f; e++) {

@@ -109,4 +109,45 @@ def cfg4_scenes(count=32, seed=SEED):
     return batch_scenes(count, lo=16384, hi=1048576, seed=seed)
 
 
-PRESETS = {"cfg0": cfg0, "cfg1": cfg1, "cfg1_rest": cfg1_rest, "cfg3": cfg3}
+def ref_beach(n=4_000_000, seed=SEED):
+    """The reference's own default scene, the one its published 4M-particle rate
+    is measured on: Fluids 5.0 example 2, "large beach front"
+    (Particles::SetupDefaultParams and SetupExampleParams,
+    fluids5.0/src/particles.cpp:158-193, 246-252; grid_density 2.0 from
+    Particles::Initialize, :86): box (0,0,0)..(500,200,500), block
+    (120,60,0)..(500,195,500), h = pradius = 0.015, sim_scale 0.006, rest density
+    200, mass 0.00020543, viscosity 0.1, internal stiffness 1.5, boundary
+    stiffness 50000 and damping 150, limits 150 / 50, gravity (0,-4.8,0), ground
+    slope 0.12, moving X- wall of amplitude 100 at frequency 2, dt 0.004.  The
+    lattice spacing is AddPointsInVolume's pdist * 0.87 / sim_scale
+    (particles.cpp:321-325) and the jitter its uniform [0, spacing) per axis
+    (here: the lattice shifted by half a spacing, +-0.5 spacing seeded)."""
+    s = dam_break(n, (0, 0, 0), (500, 200, 500), (120, 60, 0), (500, 195, 500), seed=seed, visc=0.1, int_stiff=1.5)
+    s.dt = 0.004
+    s.sim_scale = 0.006
+    s.smooth_radius = 0.015
+    s.pradius = 0.015
+    s.mass = 0.00020543
+    s.rest_density = 200.0
+    s.bound_stiff = 50000.0
+    s.bound_damp = 150.0
+    s.accel_limit = 150.0
+    s.vel_limit = 50.0
+    s.bound_slope = 0.12
+    s.bound_wall_force = 100.0
+    s.bound_wall_freq = 2.0
+    s.grid_density = 2.0
+    s.grav_amt = 1.0
+    s.grav_dir = f3((0.0, -4.8, 0.0))
+    import numpy as np
+
+    pdist = np.float32(np.float32(s.mass) / np.float32(s.rest_density)) ** np.float32(1.0 / 3.0)
+    sp = float(np.float32(pdist * np.float32(0.87)) / np.float32(s.sim_scale))
+    s.spacing = sp
+    s.jitter = 0.5
+    s.init_min = f3((120 + 0.5 * sp, 60 + 0.5 * sp, 0 + 0.5 * sp))
+    s.init_max = f3((500 + 0.5 * sp, 195 + 0.5 * sp, 500 + 0.5 * sp))
+    return s
+
+
+PRESETS = {"cfg0": cfg0, "cfg1": cfg1, "cfg1_rest": cfg1_rest, "cfg3": cfg3, "ref_beach": ref_beach}

@@ -327,3 +327,19 @@ def test_autotune_keeps_state_and_results(ps):
     finally:
         for k, t in enumerate(defaults):
             ps.SetStageBlockSize(k, t)
+
+
+def test_reference_default_scene(ps):
+    """The reference's default beach scene (cell 0.75 h under a 27-cell
+    stencil, moving X- wall, sloped floor), 20,000 particles, 3 steps:
+    integers equal, floats within 1e-5 of the FP32 oracle (the moving wall's
+    sinf may differ from the host's in the last place, so not bit-exact)."""
+    sc = scenes.ref_beach(20000)
+    ps.Restart(sc)
+    p = ps.params
+    assert p.gridSrch == 3 and p.grid_size < p.psmoothradius
+    s0 = ps.state()
+    ps.Run(3)
+    g = ps.state()
+    ref = oracle.run(p, fresh_oracle(p, s0), 3, threads=THREADS)
+    compare_state(g, ref, bitexact=False)
