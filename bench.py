@@ -20,6 +20,12 @@ the same trajectory (the step is deterministic) every --stats-every steps.
 
 CPU baseline: the FP32 CPU oracle (a port of the reference's step) on the
 host's cores, on the state at the start of the timed window.
+
+Other workloads (--workload): cfg1_rest (the same 4M particles at rest
+spacing), cfg2 (64-scene batch, with the same scenes stepped one at a time
+beside it), cfg3 / cfg4 (forward+adjoint rollouts), ref_beach (the
+reference's default scene, the only one with a published rate:
+vs_baseline = value / 6.4e8, RTX 4090, README.md:12-19).
 """
 import argparse
 import json
