@@ -258,11 +258,11 @@ int Particles::AdjStep(int record_slot) {
 int Particles::AdjForward(double* loss, double* ms) {
   FX_TRY(AdjTables());
   ReleaseGraph();
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  fx::Event e0, e1;
   if (ms) {
-    FX_HIP(hipEventCreate(&e0), "adjoint");
-    FX_HIP(hipEventCreate(&e1), "adjoint");
-    FX_HIP(hipEventRecord(e0, m_Stream), "adjoint");
+    FX_HIP(e0.create(), "adjoint");
+    FX_HIP(e1.create(), "adjoint");
+    FX_HIP(hipEventRecord(e0.e, m_Stream), "adjoint");
   }
   const int last = m_AdjSegs - 1;
   for (int t = 0; t < m_AdjSteps; t++) {
@@ -291,7 +291,7 @@ int Particles::AdjForward(double* loss, double* ms) {
   FX_HIP(hipModuleLaunchKernel(m_AdjFunc[ADJ_REDUCE], (unsigned)m_AdjScenes, 1, 1, kRedThreads, 1, 1, 0, m_Stream,
                                rargs, nullptr),
          "adjReduce");
-  if (ms) FX_HIP(hipEventRecord(e1, m_Stream), "adjoint");
+  if (ms) FX_HIP(hipEventRecord(e1.e, m_Stream), "adjoint");
   std::vector<double> acc(3 * (size_t)m_AdjScenes);
   FX_HIP(hipMemcpyAsync(acc.data(), m_AdjAccum, acc.size() * sizeof(double), hipMemcpyDeviceToHost, m_Stream),
          "adjoint");
@@ -302,10 +302,8 @@ int Particles::AdjForward(double* loss, double* ms) {
   }
   if (ms) {
     float f = 0.0f;
-    FX_HIP(hipEventElapsedTime(&f, e0, e1), "adjoint");
+    FX_HIP(hipEventElapsedTime(&f, e0.e, e1.e), "adjoint");
     *ms = f;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
   }
   m_AdjTapeValid = true;
   m_AdjLossDone = true;
@@ -316,11 +314,11 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
   if (!m_AdjLossDone) return fx::fail(FX_ESTATE, "adjoint: call fx_adjoint_forward before fx_adjoint_backward");
   FX_TRY(AdjTables());
   ReleaseGraph();
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  fx::Event e0, e1;
   if (ms) {
-    FX_HIP(hipEventCreate(&e0), "adjoint");
-    FX_HIP(hipEventCreate(&e1), "adjoint");
-    FX_HIP(hipEventRecord(e0, m_Stream), "adjoint");
+    FX_HIP(e0.create(), "adjoint");
+    FX_HIP(e1.create(), "adjoint");
+    FX_HIP(hipEventRecord(e0.e, m_Stream), "adjoint");
   }
   const size_t n = (size_t)m_AdjN;
   float* V = m_AdjVec;
@@ -382,7 +380,7 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
                                  m_Stream, rargs, nullptr),
            "adjReduce");
   }
-  if (ms) FX_HIP(hipEventRecord(e1, m_Stream), "adjoint");
+  if (ms) FX_HIP(hipEventRecord(e1.e, m_Stream), "adjoint");
   if (!first)
     FX_HIP(hipMemcpyAsync(acc.data(), m_AdjAccum, acc.size() * sizeof(double), hipMemcpyDeviceToHost, m_Stream),
            "adjoint");
@@ -397,10 +395,8 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
   }
   if (ms) {
     float f = 0.0f;
-    FX_HIP(hipEventElapsedTime(&f, e0, e1), "adjoint");
+    FX_HIP(hipEventElapsedTime(&f, e0.e, e1.e), "adjoint");
     *ms = f;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
   }
   return FX_OK;
 }

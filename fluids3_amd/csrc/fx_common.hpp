@@ -32,6 +32,18 @@ inline int hip_fail(hipError_t e, const char* where, const char* what) {
   return fail(FX_EHIP, "%s: %s failed: %s", where, what, hipGetErrorString(e));
 }
 
+// A HIP event destroyed with its scope (error returns included).
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() = default;
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  hipError_t create() { return hipEventCreate(&e); }
+};
+
 }  // namespace fx
 
 // Evaluates a HIP call; on failure records where/what and returns FX_EHIP.

@@ -631,8 +631,12 @@ int Particles::Synchronize() {
 
 int Particles::ProfileStages(int nsteps, double* ms_out) {
   if (nsteps <= 0 || ms_out == nullptr) return fx::fail(FX_EINVAL, "profile_stages: bad arguments");
+  fx::Event evs[FX_NUM_STAGES + 1];
   hipEvent_t ev[FX_NUM_STAGES + 1];
-  for (auto& e : ev) FX_HIP(hipEventCreate(&e), "profile_stages");
+  for (int k = 0; k <= FX_NUM_STAGES; k++) {
+    FX_HIP(evs[k].create(), "profile_stages");
+    ev[k] = evs[k].e;
+  }
   for (int k = 0; k <= FX_NUM_STAGES; k++) ms_out[k] = 0.0;
   int st = FX_OK;
   for (int s = 0; s < nsteps && st == FX_OK; s++) {
@@ -653,7 +657,6 @@ int Particles::ProfileStages(int nsteps, double* ms_out) {
     if (st == FX_OK && hipEventElapsedTime(&tot, ev[0], ev[FX_NUM_STAGES]) == hipSuccess)
       ms_out[FX_NUM_STAGES] += tot;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
   if (st != FX_OK) return st;
   for (int k = 0; k <= FX_NUM_STAGES; k++) ms_out[k] /= nsteps;
   return FX_OK;
@@ -798,9 +801,10 @@ int Particles::TimeSteps(int nsteps, bool use_graph, double* ms_total) {
     FX_HIP(hipStreamSynchronize(m_Stream), "time_steps");
     FX_HIP(hipMemcpyHtoD(m_Clock, &m_Params.time, sizeof(float)), "time_steps");
   }
-  hipEvent_t a, b;
-  FX_HIP(hipEventCreate(&a), "time_steps");
-  FX_HIP(hipEventCreate(&b), "time_steps");
+  fx::Event ea, eb;
+  FX_HIP(ea.create(), "time_steps");
+  FX_HIP(eb.create(), "time_steps");
+  hipEvent_t a = ea.e, b = eb.e;
   int st = FX_OK;
   if (hipEventRecord(a, m_Stream) != hipSuccess) st = fx::fail(FX_EHIP, "time_steps: record");
   if (st == FX_OK) {
@@ -819,8 +823,6 @@ int Particles::TimeSteps(int nsteps, bool use_graph, double* ms_total) {
   if (st == FX_OK && hipEventSynchronize(b) != hipSuccess) st = fx::fail(FX_EHIP, "time_steps: synchronize");
   float ms = 0.0f;
   if (st == FX_OK && hipEventElapsedTime(&ms, a, b) != hipSuccess) st = fx::fail(FX_EHIP, "time_steps: elapsed");
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
   *ms_total = ms;
   return st;
 }
