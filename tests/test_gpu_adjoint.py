@@ -73,6 +73,39 @@ def test_adjoint_dense_block_without_bitmask(ps):
     assert g["gk"][0] == pytest.approx(r["gk"], rel=1e-9)
 
 
+def test_adjoint_walls_slope_friction_moving_wall(ps):
+    """Every wall branch (sloped floor with friction, moving X- wall, X+/Z
+    walls, ceiling), initial velocities nonzero: GPU against the FP32 oracle.
+    The moving wall's sinf/cosf on the device may differ from the host's in
+    the last place, so the bar is 1e-4 norm-wise instead of bit equality."""
+    sc = scenes.dam_break(600, (-12, 0, -12), (12, 14, 12), (-11.5, 0.5, -11.5), (11.5, 13.5, 11.5))
+    sc.bound_slope = 0.12
+    sc.bound_friction = 0.3
+    sc.bound_wall_force = 3.0
+    sc.bound_wall_freq = 2.0
+    T = 12
+    P = derive_params(sc)
+    pos, clr = fill_points(sc)
+    n = pos.shape[0]
+    rng = np.random.default_rng(9)
+    v0 = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    tgt = (pos + rng.uniform(-1, 1, (n, 3))).astype(np.float32)
+    ps.Restart(sc)
+    ps.Commit(abi.FVEL, v0)
+    ps.Commit(abi.FVEVAL, v0)
+    ps.AdjointConfigure(T, 5)
+    ps.AdjointSetTarget(tgt)
+    loss, _ = ps.AdjointForward()
+    g = ps.AdjointBackward()
+    r = oracle.grad(P, {"pos": pos, "clr": clr, "vel": v0, "veval": v0}, T, tgt, fp64=False, threads=THREADS)
+    for k in ("gpos", "gvel", "gveval"):
+        e = np.linalg.norm(g[k] - r[k]) / np.linalg.norm(r[k])
+        assert e <= 1e-4, (k, e)
+    assert loss[0] == pytest.approx(r["loss"], rel=1e-5)
+    assert g["gvisc"][0] == pytest.approx(r["gvisc"], rel=1e-4)
+    assert g["gk"][0] == pytest.approx(r["gk"], rel=1e-4)
+
+
 def test_adjoint_checkpoint_spacing_invariance(ps):
     sc = scenes.scaled_dam_break(5000)
     T = 9
