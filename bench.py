@@ -21,9 +21,9 @@ the same trajectory (the step is deterministic) every --stats-every steps.
 CPU baseline: the FP32 CPU oracle (a port of the reference's step) on the
 host's cores, on the state at the start of the timed window.
 
-Other workloads (--workload): cfg1_rest (the same 4M particles at rest
-spacing), cfg2 (64-scene batch, with the same scenes stepped one at a time
-beside it), cfg3 / cfg4 (forward+adjoint rollouts), ref_beach (the
+Other workloads (--workload): dambreak_4m_rest (the same 4M particles at rest
+spacing), batch64 (64-scene batch, with the same scenes stepped one at a time
+beside it), adjoint_262k / adjoint_batch32 (forward+adjoint rollouts), ref_beach (the
 reference's default scene, the only one with a published rate:
 vs_baseline = value / 6.4e8, RTX 4090, README.md:12-19).
 """
@@ -44,8 +44,8 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (matrix and packed vector)
 # Fluids 5.0 at 4,000,000 particles on its default scene: 160 fps on an RTX 4090 (README.md:12-19)
 REF_BEACH_RTX4090 = 160 * 4_000_000
 
-TRAFFIC_FILES = {"cfg1": "traffic_dambreak_4M_literal.json", "cfg1_rest": "traffic_dambreak_4M_rest.json",
-                 "cfg2": "traffic_batch64_dambreak.json"}
+TRAFFIC_FILES = {"dambreak_4m": "traffic_dambreak_4M_literal.json", "dambreak_4m_rest": "traffic_dambreak_4M_rest.json",
+                 "batch64": "traffic_batch64_dambreak.json"}
 
 BYTES_PER_PARTICLE = {"insert": 20, "prefix": 0, "sort": 92, "pressure": 20, "force": 44, "advance": 88}
 BYTES_PER_CELL = {"insert": 4, "prefix": 8, "sort": 4, "pressure": 0, "force": 0, "advance": 0}
@@ -73,12 +73,13 @@ def describe(name, sc, N, G):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (200; 100 for cfg2)")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (200; 100 for batch64)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="cfg1",
-                    choices=["cfg0", "cfg1", "cfg1_rest", "cfg2", "cfg3", "cfg4", "ref_beach"])
+    ap.add_argument("--workload", default="dambreak_4m",
+                    choices=["dambreak_16k", "dambreak_4m", "dambreak_4m_rest", "batch64", "adjoint_262k",
+                             "adjoint_batch32", "ref_beach"])
     ap.add_argument("--adjoint-steps", type=int, default=20,
-                    help="cfg1: steps of the forward+adjoint rollout timed beside the plain step (0 = skip)")
+                    help="dambreak_4m: steps of the forward+adjoint rollout timed beside the plain step (0 = skip)")
     ap.add_argument("--checkpoint-every", type=int, default=10)
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default block sizes instead of tuning them after the warm-up")
@@ -91,7 +92,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every core this process may use")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-single", action="store_true", help="cfg2: skip the one-scene-at-a-time comparison")
+    ap.add_argument("--no-single", action="store_true", help="batch64: skip the one-scene-at-a-time comparison")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (reference for the FETCH_SIZE / WRITE_SIZE "
                          "counters of a kernel of known traffic) and exit")
@@ -116,10 +117,10 @@ def main():
 
     from fluids3_amd import Particles, scenes
 
-    if args.workload in ("cfg3", "cfg4"):
+    if args.workload in ("adjoint_262k", "adjoint_batch32"):
         return adjoint_bench(args)
 
-    batch = args.workload == "cfg2"
+    batch = args.workload == "batch64"
     K = args.steps if args.steps is not None else (100 if batch else 200)
     W = args.warmup
     ps = Particles() if args.code_object is None else Particles(args.code_object)
@@ -240,7 +241,7 @@ def main():
 
     # forward+adjoint rollout beside the plain step, same start state and window
     adjoint = None
-    if args.workload == "cfg1" and args.adjoint_steps > 0:
+    if args.workload == "dambreak_4m" and args.adjoint_steps > 0:
         adjoint = adjoint_ratio(ps, restart, W, args.adjoint_steps, args.checkpoint_every, use_graph)
 
     cpu = None
@@ -314,30 +315,30 @@ def adjoint_ratio(ps, restart, W, T, K, use_graph):
 
 
 def adjoint_bench(args):
-    """cfg3: one differentiable dam break; cfg4: a batch of 32.  value =
+    """adjoint_262k: one differentiable dam break; adjoint_batch32: a batch of 32.  value =
     particle-steps/s of the forward+adjoint rollout (loss and gradients with
     respect to initial velocities, viscosity and stiffness)."""
     from fluids3_amd import Particles, scenes
     from fluids3_amd.rollout import target_positions
 
     ps = Particles()
-    if args.workload == "cfg3":
-        sc = scenes.cfg3()
+    if args.workload == "adjoint_262k":
+        sc = scenes.dambreak_262k()
         T = args.steps if args.steps is not None else 50
         K = args.checkpoint_every
         target = target_positions(ps, sc, T)
         restart = lambda: ps.Restart(sc)
-        desc = (f"cfg3: differentiable dam break, {sc.num_points:,} particles, {T} steps, checkpoints every {K}, "
+        desc = (f"adjoint_262k: differentiable dam break, {sc.num_points:,} particles, {T} steps, checkpoints every {K}, "
                 f"target = the rollout at viscosity 0.30 and stiffness 1.2")
     else:
-        sc = scenes.cfg4_scenes()
+        sc = scenes.batch32_scenes()
         T = args.steps if args.steps is not None else 40
         K = args.checkpoint_every if args.checkpoint_every != 10 else 8
         # each scene fits its viscosity to the rollout at viscosity 0.35
         target = np.concatenate([target_positions(ps, s, T, visc=0.35, k=s.int_stiff) for s in sc])
         restart = lambda: ps.RestartBatch(sc)
         n = [s.num_points for s in sc]
-        desc = (f"cfg4: {len(sc)} differentiable dam breaks in one batch, {sum(n):,} particles "
+        desc = (f"adjoint_batch32: {len(sc)} differentiable dam breaks in one batch, {sum(n):,} particles "
                 f"({min(n):,}..{max(n):,} per scene), {T} steps, checkpoints every {K}, "
                 f"target = each scene's rollout at viscosity 0.35")
     restart()

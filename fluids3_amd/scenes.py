@@ -58,12 +58,12 @@ def rest_spacing():
     return (0.00020543 / 600.0) ** (1.0 / 3.0) / 0.004
 
 
-def cfg0(n=16384, seed=SEED):
+def dambreak_16k(n=16384, seed=SEED):
     """16,384 particles; box (-30,0,-30)..(30,60,30), block (-25,5,-25)..(0,45,25)."""
     return dam_break(n, (-30, 0, -30), (30, 60, 30), (-25, 5, -25), (0, 45, 25), seed=seed)
 
 
-def cfg1(n=4_000_000, seed=SEED):
+def dambreak_4m(n=4_000_000, seed=SEED):
     """4,000,000 particles; box (-120,0,-60)..(120,120,60), block = left 40% in x.
 
     At the stated size the block holds the particles at 0.40x rest spacing
@@ -84,8 +84,8 @@ def scaled_dam_break(n, seed=SEED, visc=0.35, int_stiff=1.0, fill=0.4):
     return dam_break(n, bmin, bmax, bmin, blk_max, seed=seed, visc=visc, int_stiff=int_stiff)
 
 
-def cfg1_rest(n=4_000_000, seed=SEED):
-    """4,000,000 particles at rest spacing: the cfg1 box scaled up until its left 40% holds them."""
+def dambreak_4m_rest(n=4_000_000, seed=SEED):
+    """4,000,000 particles at rest spacing: the dambreak_4m box scaled up until its left 40% holds them."""
     return scaled_dam_break(n, seed=seed)
 
 
@@ -99,12 +99,12 @@ def batch_scenes(count=64, lo=8192, hi=262144, seed=SEED, visc_range=(0.2, 0.5))
     return [scaled_dam_break(int(n), seed=seed + 1 + k, visc=float(v)) for k, (n, v) in enumerate(zip(ns, viscs))]
 
 
-def cfg3(n=262144, seed=SEED):
+def dambreak_262k(n=262144, seed=SEED):
     """Differentiable dam break: 262,144 particles at rest spacing, fill 0.4."""
     return scaled_dam_break(n, seed=seed)
 
 
-def cfg4_scenes(count=32, seed=SEED):
+def batch32_scenes(count=32, seed=SEED):
     """Batched adjoint: counts log-uniform in [16,384, 1,048,576], viscosity uniform in [0.2, 0.5]."""
     return batch_scenes(count, lo=16384, hi=1048576, seed=seed)
 
@@ -150,4 +150,5 @@ def ref_beach(n=4_000_000, seed=SEED):
     return s
 
 
-PRESETS = {"cfg0": cfg0, "cfg1": cfg1, "cfg1_rest": cfg1_rest, "cfg3": cfg3, "ref_beach": ref_beach}
+PRESETS = {"dambreak_16k": dambreak_16k, "dambreak_4m": dambreak_4m, "dambreak_4m_rest": dambreak_4m_rest,
+           "dambreak_262k": dambreak_262k, "ref_beach": ref_beach}

@@ -3,7 +3,7 @@
 // CPU-baseline leg of bench.py.
 //
 // Parity status of each function is pinned by tests/test_oracle.py:
-//   orc_derive            grid + kernel constants   pinned: closed-form cfg0/cfg1
+//   orc_derive            grid + kernel constants   pinned: closed-form 16k/4M dam-break
 //                                                     grid numbers, gridAdj formula
 //   orc_run (insert)      cell, counts              pinned: hand-placed particles,
 //                                                     brute-force counts

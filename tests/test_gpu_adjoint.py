@@ -218,7 +218,7 @@ def test_autograd_function_carries_the_adjoint(ps):
 
 
 @pytest.mark.slow
-def test_adjoint_cfg3_full_size(ps):
+def test_adjoint_dambreak_262k_full_size(ps):
     """262,144 particles, 50 steps, checkpoints every 10.
     - The FP32 GPU gradient against the FP64 adjoint over the same FP32 forward
       (tape promoted to double), every component: bar 1e-5.
@@ -239,19 +239,19 @@ def test_adjoint_cfg3_full_size(ps):
     mx = oracle.grad_mixed(P, {"pos": pos, "clr": clr}, T, target, threads=THREADS)
     for k in ("gpos", "gvel", "gveval"):
         e = np.linalg.norm(g[k] - mx[k]) / np.linalg.norm(mx[k])
-        print(f"cfg3 {k}: FP32 GPU vs FP64 adjoint of the same forward {e:.2e}")
+        print(f"dambreak_262k {k}: FP32 GPU vs FP64 adjoint of the same forward {e:.2e}")
         assert e <= 1e-5, k
     for k in ("gvisc", "gk"):
         e = abs(g[k][0] - mx[k]) / abs(mx[k])
-        print(f"cfg3 {k}: FP32 GPU {g[k][0]:.8e} vs FP64 adjoint of the same forward {mx[k]:.8e} ({e:.2e})")
+        print(f"dambreak_262k {k}: FP32 GPU {g[k][0]:.8e} vs FP64 adjoint of the same forward {mx[k]:.8e} ({e:.2e})")
         assert e <= 1e-5, k
     visc0, k0 = float(np.float32(sc.visc)), float(np.float32(sc.int_stiff))
     r = oracle.grad(P, {"pos": pos.astype(np.float64), "clr": clr}, T, target.astype(np.float64), fp64=True,
                     coef=(visc0, k0), threads=THREADS)
     for k in ("gpos", "gvel", "gveval"):
-        print(f"cfg3 {k}: FP32 GPU vs FP64 rollout's adjoint {np.linalg.norm(g[k] - r[k]) / np.linalg.norm(r[k]):.2e}")
+        print(f"dambreak_262k {k}: FP32 GPU vs FP64 rollout's adjoint {np.linalg.norm(g[k] - r[k]) / np.linalg.norm(r[k]):.2e}")
     for k in ("gvisc", "gk"):
-        print(f"cfg3 {k}: FP32 GPU {g[k][0]:.8e}, FP64 rollout's adjoint {r[k]:.8e}")
+        print(f"dambreak_262k {k}: FP32 GPU {g[k][0]:.8e}, FP64 rollout's adjoint {r[k]:.8e}")
     base = {"pos": pos.astype(np.float64), "clr": clr}
     tgt = target.astype(np.float64)
     n = sc.num_points
@@ -284,7 +284,7 @@ def test_adjoint_cfg3_full_size(ps):
         if abs(d1 - d2) > 1e-5 * max(abs(d1), abs(d2), 1e-3 * scale):
             continue  # a branch switch inside the step: take another component
         e = abs(rv0[i, a] - d2) / max(abs(d2), 1e-3 * scale)
-        print(f"cfg3 dL/dv0[{i},{a}]: FP64 adjoint {rv0[i, a]:.8e}, central difference {d2:.8e}, rel {e:.2e}")
+        print(f"dambreak_262k dL/dv0[{i},{a}]: FP64 adjoint {rv0[i, a]:.8e}, central difference {d2:.8e}, rel {e:.2e}")
         errs.append(e)
     assert len(errs) == 6
     assert max(errs) <= 1e-4

@@ -37,7 +37,7 @@ def ps():
 
 
 def test_save_load_round_trip(ps, tmp_path):
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     ps.Restart(sc)
     ps.Run(3)
     st = ps.state()
@@ -63,7 +63,7 @@ def test_save_load_round_trip(ps, tmp_path):
 
 
 def test_load_fluids40_file_and_count_mismatch(ps, tmp_path):
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     ps.Restart(sc)
     st = ps.state()
     n = sc.num_points
@@ -89,7 +89,7 @@ def test_sample_particles_against_oracle(ps):
     from oracle import oracle
     from parity import rel_err
 
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     ps.Restart(sc)
     ps.Run(3)
     st = ps.state()

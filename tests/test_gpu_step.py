@@ -52,8 +52,8 @@ def compare_state(g, ref, floats=("press", "force", "pos", "vel", "veval"), bite
             assert bits_equal(a, b), f"{k}: not bit-identical to the FP32 oracle (max rel {rel_err(a, b):.2e})"
 
 
-def test_one_step_cfg0(ps):
-    sc = scenes.cfg0()
+def test_one_step_dambreak_16k(ps):
+    sc = scenes.dambreak_16k()
     ps.Restart(sc)
     p = ps.params
     s0 = ps.state()
@@ -76,9 +76,9 @@ def test_one_step_cfg0(ps):
         assert e.max() < (1e-5 if k in ("press", "pos") else 1e-2), k
 
 
-def test_stages_cfg0(ps):
+def test_stages_dambreak_16k(ps):
     """Each per-function stage against the oracle, fed the device state before it."""
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     ps.Restart(sc)
     p = ps.params
     s0 = ps.state()
@@ -119,8 +119,8 @@ def test_stages_cfg0(ps):
             assert bits_equal(a, b), k
 
 
-def test_multi_step_bitexact_cfg0(ps):
-    sc = scenes.cfg0()
+def test_multi_step_bitexact_dambreak_16k(ps):
+    sc = scenes.dambreak_16k()
     ps.Restart(sc)
     p = ps.params
     s0 = ps.state()
@@ -131,7 +131,7 @@ def test_multi_step_bitexact_cfg0(ps):
 
 
 def test_determinism(ps):
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     out = []
     for _ in range(2):
         ps.Restart(sc)
@@ -144,7 +144,7 @@ def test_determinism(ps):
 
 
 def test_graph_equals_eager(ps):
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     ps.Restart(sc)
     ps.Run(6)
     eager = ps.state()
@@ -166,7 +166,7 @@ def test_graph_equals_eager(ps):
 def test_block_size_invariance(ps, threads):
     """Results do not depend on the launch geometry (the force stage switches
     from its pair ring to in-place evaluation above 64 threads per block)."""
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     defaults = [ps.StageBlockSize(k) for k in range(abi.NUM_STAGES)]
     ps.Restart(sc)
     ps.Run(3)
@@ -185,7 +185,7 @@ def test_block_size_invariance(ps, threads):
 
 @pytest.mark.parametrize("n", [1, 2, 63, 65, 1000, 4097])
 def test_ragged_counts(ps, n):
-    sc = scenes.cfg0(n=n)
+    sc = scenes.dambreak_16k(n=n)
     ps.Restart(sc)
     p = ps.params
     s0 = ps.state()
@@ -198,7 +198,7 @@ def test_ragged_counts(ps, n):
 def test_out_of_range_and_nan(ps):
     """GRID_UNDEF particles: counted in the tail slot, sorted after every cell in
     particle order, parked at (-1000)^3 with zero velocity."""
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     ps.Restart(sc)
     p = ps.params
     s0 = ps.state()
@@ -225,7 +225,7 @@ def test_large_tail_bucket_is_stable(ps):
     """More out-of-range particles than a cell's member ranking covers: the
     tail bucket is still ordered by particle index (flag scan), so runs are
     reproducible and equal to the oracle, ids included."""
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     ps.Restart(sc)
     p = ps.params
     pos = ps.state()["pos"].copy()
@@ -254,10 +254,10 @@ def test_scan_lengths(ps, n):
     assert_int_equal("scan", got, want)
 
 
-def test_cfg1_full_size(ps):
+def test_dambreak_4m_full_size(ps):
     """4,000,000 particles: size-independent properties of the sort, and the
     density and force stages against the oracle on the same device state."""
-    sc = scenes.cfg1()
+    sc = scenes.dambreak_4m()
     ps.Restart(sc)
     p = ps.params
     n = ps.NumPoints()
@@ -303,7 +303,7 @@ def test_autotune_keeps_state_and_results(ps):
     started from, and a rollout under the tuned sizes is bit-identical to
     one under the default sizes."""
     defaults = [ps.StageBlockSize(k) for k in range(abi.NUM_STAGES)]
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     try:
         ps.Restart(sc)
         ps.Run(3)

@@ -44,16 +44,16 @@ def test_code_object_exports_the_module_contract():
 def test_params_layout_688_bytes():
     assert C.sizeof(abi.FParams) == 688
     assert abi.FParams.gridAdj.offset == 240 and abi.FParams.pmem.offset == 684
-    P = derive_params(scenes.cfg0())
+    P = derive_params(scenes.dambreak_16k())
     assert P.gridRes.tolist() == [29, 29, 29] and P.pnum == 16384
 
 
 def test_derive_rejects_bad_scenes():
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     sc.num_points = 0
     with pytest.raises(FluidsError):
         derive_params(sc)
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     sc.grid_density = 0.3  # cell of 1.5h/0.3 = 5h: fine; 30 makes the search exceed gridAdj
     derive_params(sc)
     sc.grid_density = 30.0
@@ -62,7 +62,7 @@ def test_derive_rejects_bad_scenes():
 
 
 def test_fill_is_deterministic_and_in_block():
-    sc = scenes.cfg1()
+    sc = scenes.dambreak_4m()
     sc.num_points = 200_000
     pos, clr = fill_points(sc)
     assert pos.shape == (200_000, 3)

@@ -42,8 +42,8 @@ def cell_of(p, P):
 # ------------------------------------------------------------------ grid
 
 
-def test_grid_cfg0_known_values():
-    P = params_for(scenes.cfg0())
+def test_grid_dambreak_16k_known_values():
+    P = params_for(scenes.dambreak_16k())
     assert P.gridRes.tolist() == [29, 29, 29] and P.gridTotal == 24389
     assert P.gridSrch == 3 and P.gridAdjCnt == 27 and P.gridScanMax.tolist() == [26, 26, 26]
     # grid_size / sim_scale in float32 is 2.4999998, so gridMin.y = 0 - 2 * 2.4999998
@@ -52,14 +52,14 @@ def test_grid_cfg0_known_values():
     assert np.array(P.gridMin.tolist(), F32).tolist() == [F32(-35.0), F32(-4.9999995), F32(-35.0)]
 
 
-def test_grid_cfg1_known_values():
-    P = params_for(scenes.cfg1())
+def test_grid_dambreak_4m_known_values():
+    P = params_for(scenes.dambreak_4m())
     assert P.gridRes.tolist() == [101, 53, 53] and P.gridTotal == 283709
     assert P.gridScanMax.tolist() == [98, 50, 50]
 
 
 def test_grid_adjacency_and_oracle_derivation_agree():
-    for sc in (scenes.cfg0(), scenes.cfg1(), scenes.cfg1_rest()):
+    for sc in (scenes.dambreak_16k(), scenes.dambreak_4m(), scenes.dambreak_4m_rest()):
         P = params_for(sc)
         Q = oracle.derive(P)
         assert bytes(P) == bytes(Q), "host and oracle derivations differ"
@@ -69,7 +69,7 @@ def test_grid_adjacency_and_oracle_derivation_agree():
 
 
 def test_kernel_constants_closed_form():
-    P = params_for(scenes.cfg0())
+    P = params_for(scenes.dambreak_16k())
     h, ss = 0.01, 0.004
     assert math.isclose(P.poly6kern, 315.0 / (64.0 * 3.141592 * h ** 9), rel_tol=2e-6)
     assert math.isclose(P.spikykern, -45.0 / (3.141592 * h ** 6), rel_tol=2e-6)
@@ -112,7 +112,7 @@ def test_insert_hand_placed_particles():
 
 
 def test_insert_matches_float32_cell_formula_and_counts():
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     P = params_for(sc)
     rng = np.random.default_rng(1)
     # positions over the whole grid, including outside the scan range
@@ -135,15 +135,15 @@ def test_insert_nan_is_out_of_range():
 
 
 def test_scan_is_exclusive_cumsum():
-    P = params_for(scenes.cfg0())
-    pos, _ = fill_points(scenes.cfg0())
+    P = params_for(scenes.dambreak_16k())
+    pos, _ = fill_points(scenes.dambreak_16k())
     st = oracle.run(P, oracle.make_state(P, pos), 1, oracle.STAGE_INSERT | oracle.STAGE_SCAN)
     want = np.concatenate([[0], np.cumsum(st["cnt"].astype(np.int64))[:-1]])
     assert np.array_equal(st["off"], want.astype(np.uint32))
 
 
 def test_sort_is_stable_sort_by_cell():
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     P = params_for(sc)
     pos, clr = fill_points(sc)
     rng = np.random.default_rng(3)
@@ -208,7 +208,7 @@ def test_density_cubic_lattice_shell_sum():
 
 
 def test_density_grid_equals_brute_force():
-    sc = scenes.cfg0(n=3000)
+    sc = scenes.dambreak_16k(n=3000)
     P = params_for(sc)
     pos, _ = fill_points(sc)
     st = oracle.run(P, oracle.make_state(P, pos), 1,
@@ -356,7 +356,7 @@ def test_advance_parks_out_of_range_particles():
 
 
 def test_fp32_and_fp64_builds_agree_closely():
-    sc = scenes.cfg0(n=4000)
+    sc = scenes.dambreak_16k(n=4000)
     P = params_for(sc)
     pos, _ = fill_points(sc)
     a = oracle.run(P, oracle.make_state(P, pos), 1)
@@ -368,7 +368,7 @@ def test_fp32_and_fp64_builds_agree_closely():
 
 
 def test_lattice_fill_rule():
-    sc = scenes.cfg0()
+    sc = scenes.dambreak_16k()
     s_rest = scenes.rest_spacing()
     s = scene_spacing(sc)
     assert s < s_rest  # 16,384 do not fit the block at rest spacing
@@ -379,5 +379,5 @@ def test_lattice_fill_rule():
     assert np.abs(pos - (lo + k * s)).max() <= 0.1 * s * 1.0001  # jitter within +-0.1 spacing
     pos2, _ = fill_points(sc)
     assert np.array_equal(pos, pos2)
-    other = scenes.cfg0(seed=sc.seed + 1)
+    other = scenes.dambreak_16k(seed=sc.seed + 1)
     assert not np.array_equal(pos, fill_points(other)[0])
