@@ -45,7 +45,7 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 (matrix and packed vector)
 REF_BEACH_RTX4090 = 160 * 4_000_000
 
 TRAFFIC_FILES = {"dambreak_4m": "traffic_dambreak_4M_literal.json", "dambreak_4m_rest": "traffic_dambreak_4M_rest.json",
-                 "batch64": "traffic_batch64_dambreak.json"}
+                 "batch64": "traffic_batch64_dambreak.json", "ref_beach": "traffic_ref_beach_4M.json"}
 
 BYTES_PER_PARTICLE = {"insert": 20, "prefix": 0, "sort": 92, "pressure": 20, "force": 44, "advance": 88}
 BYTES_PER_CELL = {"insert": 4, "prefix": 8, "sort": 4, "pressure": 0, "force": 0, "advance": 0}
@@ -258,7 +258,10 @@ def main():
                          f"of the timed window on {threads} host threads ({sec:.2f} s)"}
 
     out = {
-        "metric": ("particle-steps/sec at 4M particles (1 GPU)" if N == 4_000_000 and not batch
+        # the headline workload carries the forward+adjoint ratio in the same line ("adjoint")
+        "metric": ("particle-steps/sec at 4M particles (1 GPU); fwd+adjoint step time vs plain"
+                   if args.workload == "dambreak_4m" and adjoint is not None
+                   else "particle-steps/sec at 4M particles (1 GPU)" if N == 4_000_000 and not batch
                    else "particle-steps/sec"),
         "value": round(value, 1),
         "unit": "particle-steps/s",

@@ -1,4 +1,4 @@
-"""Builds the gfx950 code object and libfluidsx.so in-tree (fluids3_amd/).
+"""Builds the gfx950 code objects (the step, the adjoint) and libfluidsx.so in-tree (fluids3_amd/).
 
 Run as `python -m fluids3_amd.build`.  Always recompiles.
 """
@@ -17,7 +17,7 @@ def build(verbose=False):
     if r.returncode != 0:
         sys.stderr.write((r.stdout or "") + (r.stderr or ""))
         raise RuntimeError("building fluids3_amd failed")
-    for f in ("particles_gfx950.hsaco", "libfluidsx.so"):
+    for f in ("particles_gfx950.hsaco", "adjoint_gfx950.hsaco", "libfluidsx.so"):
         if not os.path.exists(os.path.join(HERE, f)):
             raise RuntimeError(f"build did not produce {f}")
 
