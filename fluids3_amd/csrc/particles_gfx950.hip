@@ -155,9 +155,10 @@ extern "C" __global__ void fxFillUint(uint* __restrict__ p, uint v, int n) {
 // indices are offsets into the concatenated cell table (c0 + local cell).
 //
 // The counting is aggregated per wave: the particles arrive sorted by cell
-// from the previous step, so the 64 lanes of a wave fall into a few cells;
+// from the previous step, so the 64 lanes of a wave fall into few cells;
 // lanes of one cell take a single atomicAdd of their number and their slot
-// is the returned base plus their rank among those lanes.
+// is the returned base plus their rank among those lanes.  The atomics of
+// all the wave's cells are issued together (no round trip per cell).
 __device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S) {
   const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
   const bool active = i < S.p1;
@@ -180,19 +181,27 @@ __device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S) {
     if (tflag != nullptr) tflag[i] = in ? 0u : 1u;
   }
   uint* cnt = pbuf<uint>(FAccel, AGRIDCNT);
+  // group the wave's lanes by slot (register-only ballots), then every
+  // group's first lane adds the group's size with one atomic; all groups'
+  // atomics go out together and each lane reads its group's base from the
+  // leader lane
   unsigned long long todo = __ballot(active);
-  uint slot = 0;
+  unsigned long long mine = 0ull;
+  int leader_of = 0;
   while (todo != 0ull) {  // one pass per distinct slot in the wave
     const int leader = __ffsll((long long)todo) - 1;
     const int v = __builtin_amdgcn_readlane(gs, leader);
     const unsigned long long same = __ballot(active && gs == v) & todo;
-    uint base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(&cnt[v], (uint)__popcll(same));
-    base = (uint)__builtin_amdgcn_readlane((int)base, leader);
-    if (active && gs == v) slot = base + lanes_below(same);
+    if (active && gs == v) {
+      mine = same;
+      leader_of = leader;
+    }
     todo &= ~same;
   }
-  if (active) pbuf<int>(FPnts, FGNDX)[i] = (int)slot;
+  uint base = 0;
+  if (active && (int)__lane_id() == leader_of) base = atomicAdd(&cnt[gs], (uint)__popcll(mine));
+  base = (uint)__shfl((int)base, leader_of, 64);
+  if (active) pbuf<int>(FPnts, FGNDX)[i] = (int)(base + lanes_below(mine));
 }
 
 extern "C" __global__ void insertParticles(int pnum) {
