@@ -306,17 +306,33 @@ __device__ __forceinline__ void sort_full_impl(const FParams_t& P, const Span S)
     // tail bucket: the number of the scene's out-of-range particles before i
     rank = (int)(tscan[i] - tscan[S.p0]);
   } else if (n <= FX_RANK_CAP) {
+    // members with a smaller pre-sort index; eight loads in flight per step
     rank = 0;
-    for (uint k = 0; k < n; k++) rank += (grid[o + k] < (uint)i) ? 1 : 0;
+    uint k = 0;
+    for (; k + 8 <= n; k += 8) {
+      uint g[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) g[u] = grid[o + k + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) rank += (g[u] < (uint)i) ? 1 : 0;
+    }
+    for (; k < n; k++) rank += (grid[o + k] < (uint)i) ? 1 : 0;
   } else {
     rank = pbuf<const int>(FPntTmp, FGNDX)[i];
   }
+  // every load before the first store (the two sets are distinct buffers,
+  // which the compiler cannot know)
+  const fx_float3 pos = pbuf<const fx_float3>(FPntTmp, FPOS)[i];
+  const fx_float3 vel = pbuf<const fx_float3>(FPntTmp, FVEL)[i];
+  const fx_float3 vev = pbuf<const fx_float3>(FPntTmp, FVEVAL)[i];
+  const uint clr = pbuf<const uint>(FPntTmp, FCLR)[i];
+  const uint id = pbuf<const uint>(FPntTmp, FID)[i];
   const uint s = o + (uint)rank;
-  pbuf<fx_float3>(FPnts, FPOS)[s] = pbuf<const fx_float3>(FPntTmp, FPOS)[i];
-  pbuf<fx_float3>(FPnts, FVEL)[s] = pbuf<const fx_float3>(FPntTmp, FVEL)[i];
-  pbuf<fx_float3>(FPnts, FVEVAL)[s] = pbuf<const fx_float3>(FPntTmp, FVEVAL)[i];
-  pbuf<uint>(FPnts, FCLR)[s] = pbuf<const uint>(FPntTmp, FCLR)[i];
-  pbuf<uint>(FPnts, FID)[s] = pbuf<const uint>(FPntTmp, FID)[i];
+  pbuf<fx_float3>(FPnts, FPOS)[s] = pos;
+  pbuf<fx_float3>(FPnts, FVEL)[s] = vel;
+  pbuf<fx_float3>(FPnts, FVEVAL)[s] = vev;
+  pbuf<uint>(FPnts, FCLR)[s] = clr;
+  pbuf<uint>(FPnts, FID)[s] = id;
   pbuf<int>(FPnts, FGCELL)[s] = c;
   pbuf<int>(FPnts, FGNDX)[s] = rank;
 }
