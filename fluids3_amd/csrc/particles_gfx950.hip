@@ -526,12 +526,14 @@ extern "C" __global__ __launch_bounds__(1024) void computePressureBatch(BatchArg
 // F += (pterm * dist + vterm * (veval_j - veval_i)) * c / (rho_i rho_j).
 //
 // With the density stage's bitmask the kernel never re-tests candidates.
-// Each lane turns the set bits of its words into neighbour indices in a
-// per-lane ring in LDS; the wave then evaluates queued pairs in batches: a
-// batch takes from every lane as many pairs as the shortest active queue
-// holds (more only when a queue must make room), so the expensive pair term
-// runs with nearly every lane busy.  Each lane adds its own pairs in
-// candidate order.  Waves are independent: no block-wide barrier.
+// Two walks over the bitmask: the word queue further below (blocks of up to
+// 256 threads, 2- and 3-cell stencil rows), and the pair ring here (any
+// block size and stencil).  The ring: each lane turns the set bits of its
+// words into neighbour indices in a per-lane ring in LDS; the wave then
+// evaluates queued pairs in batches: a batch takes from every lane as many
+// pairs as the shortest active queue holds (more only when a queue must make
+// room).  Each lane adds its own pairs in candidate order.  Waves are
+// independent: no block-wide barrier.
 #define FQ_SLOTS 1024  // ring slots per block (16 per lane at 64 threads)
 
 struct ForceI {
@@ -545,21 +547,52 @@ struct ForceI {
 //   F += A dist + B (veval_j - veval_i),
 //   A = ((ss * -0.5) * spiky) c^2 (p_i + p_j) inv,  B = vterm c r inv,
 // which is the reference's (pterm dist + vterm dv) c / D with
-// pterm = ss (-0.5) c spiky (p_i + p_j) / r.
-__device__ __forceinline__ void force_pair(const FParams_t& P, ForceI& I, const float4 a, const float4 v) {
-  const float dx = I.px - a.x, dy = I.py - a.y, dz = I.pz - a.z;
-  const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+// pterm = ss (-0.5) c spiky (p_i + p_j) / r.  The term is split from its
+// accumulation so that several pairs' terms can be evaluated before they
+// are added in order.
+struct PairTerm {
+  float A, B, dx, dy, dz, ux, uy, uz;
+};
+
+__device__ __forceinline__ PairTerm pair_term(const FParams_t& P, const ForceI& I, const float4 a,
+                                              const float4 v) {
+  PairTerm q;
+  q.dx = I.px - a.x;
+  q.dy = I.py - a.y;
+  q.dz = I.pz - a.z;
+  const float dsq = fmaf(q.dz, q.dz, fmaf(q.dy, q.dy, q.dx * q.dx));
   const float rr = sqrtf(dsq * P.d2);
   const float c = P.psmoothradius - rr;
   const float dj = a.w;
   const float pj = (dj - P.prest_dens) * P.pintstiff;
   const float inv = 1.0f / (rr * (I.di * dj));
   const float kp = (P.sim_scale * -0.5f) * P.spikykern;
-  const float A = (kp * (c * c)) * (I.pi + pj) * inv;
-  const float Bv = (P.vterm * c) * rr * inv;
-  I.fx = fmaf(A, dx, fmaf(Bv, v.x - I.vx, I.fx));
-  I.fy = fmaf(A, dy, fmaf(Bv, v.y - I.vy, I.fy));
-  I.fz = fmaf(A, dz, fmaf(Bv, v.z - I.vz, I.fz));
+  q.A = (kp * (c * c)) * (I.pi + pj) * inv;
+  q.B = (P.vterm * c) * rr * inv;
+  q.ux = v.x - I.vx;
+  q.uy = v.y - I.vy;
+  q.uz = v.z - I.vz;
+  return q;
+}
+
+__device__ __forceinline__ void pair_add(ForceI& I, const PairTerm& q) {
+  I.fx = fmaf(q.A, q.dx, fmaf(q.B, q.ux, I.fx));
+  I.fy = fmaf(q.A, q.dy, fmaf(q.B, q.uy, I.fy));
+  I.fz = fmaf(q.A, q.dz, fmaf(q.B, q.uz, I.fz));
+}
+
+// pair_add when ok, else nothing (a select, no branch)
+__device__ __forceinline__ void pair_add_if(ForceI& I, const PairTerm& q, bool ok) {
+  const float fx = fmaf(q.A, q.dx, fmaf(q.B, q.ux, I.fx));
+  const float fy = fmaf(q.A, q.dy, fmaf(q.B, q.uy, I.fy));
+  const float fz = fmaf(q.A, q.dz, fmaf(q.B, q.uz, I.fz));
+  I.fx = ok ? fx : I.fx;
+  I.fy = ok ? fy : I.fy;
+  I.fz = ok ? fz : I.fz;
+}
+
+__device__ __forceinline__ void force_pair(const FParams_t& P, ForceI& I, const float4 a, const float4 v) {
+  pair_add(I, pair_term(P, I, a, v));
 }
 
 __device__ __forceinline__ bool is_pair(const FParams_t& P, const ForceI& I, const float4 a) {
@@ -659,10 +692,7 @@ __device__ __forceinline__ void force_impl(const FParams_t& P, const Span S, int
   int W = 0;
   // evaluates the ring's first f entries of every lane (fewer if it holds
   // fewer); a lane whose ring is empty evaluates record 0 and discards the
-  // result.  The pair gathers of this loop are what bounds the kernel: the
-  // vector-memory return path is busy nearly all the time, and neither
-  // deeper rings (fewer waves fit) nor loading several entries' records
-  // ahead of their evaluation (more idle lanes) made it faster.
+  // result.
   auto drain = [&](int f) {
 #pragma unroll 2
     for (int e = 0; e < f; e++) {
@@ -713,16 +743,206 @@ __device__ __forceinline__ void force_impl(const FParams_t& P, const Span S, int
   if (act) pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{I.fx, I.fy, I.fz};
 }
 
+#define FW_AHEAD 4  // bitmask words loaded ahead of their use
+
+// Word-queue force.  Each lane reads its bitmask words as one stream (row
+// r's words follow row r-1's) and queues every non-zero word whole, with the
+// record index of its bit 0: two words in registers, up to K more in LDS.
+// The wave then evaluates pairs in batches: a batch takes the same number of
+// pairs from every lane (as many as the shortest queue holds, more only when
+// a queue must make room), each lane's pairs in candidate order, two pairs'
+// terms evaluated at a time.  Queuing words instead of pairs costs one LDS
+// write per word, and a deep queue of words keeps nearly every lane busy in
+// the batches, so nearly every pair gather carries a pair.
+#ifndef FW_QBYTES
+#define FW_QBYTES 8192  // LDS queue per block: (word, base) pairs
+#endif
+
+template <int R, int S>
+__device__ __forceinline__ void force_walk_q(const FParams_t& P, ForceI& I, const uint* __restrict__ bits,
+                                             size_t NN, int i, int nwords, int (&E_)[R], int (&G)[R], uint* Qw,
+                                             uint* Qb, int K, const float4* srcA, const float4* srcV, bool use) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  uint cur = 0u, nxt = 0u;  // current word (bits not yet taken) and the next one
+  int curb = 0, nxtb = 0;   // their bases
+  int qh = 0, qc = 0;       // LDS queue head and length
+  int qn = 0;               // pairs held
+  // the lane's next pair, in candidate order; false when it holds none
+  auto take = [&](uint& j) -> bool {
+    const bool ok = cur != 0u;
+    j = ok ? (uint)(curb + __builtin_ctz(cur)) : 0u;
+    cur &= cur - 1u;
+    if (ok && cur == 0u) {
+      cur = nxt;
+      curb = nxtb;
+      nxt = 0u;
+      if (qc > 0) {
+        nxt = Qw[qh * nt + t];
+        nxtb = (int)Qb[qh * nt + t];
+        qh = (qh + 1 == K) ? 0 : qh + 1;
+        qc--;
+      }
+    }
+    return ok;
+  };
+  // f pairs from every lane (a lane that runs out evaluates record 0 and
+  // discards it)
+  auto drain = [&](int f) {
+    int e = 0;
+    for (; e + 2 <= f; e += 2) {
+      uint j0, j1;
+      const bool ok0 = take(j0);
+      const bool ok1 = take(j1);
+      const PairTerm q0 = pair_term(P, I, srcA[S * (size_t)j0], srcV[S * (size_t)j0]);
+      const PairTerm q1 = pair_term(P, I, srcA[S * (size_t)j1], srcV[S * (size_t)j1]);
+      pair_add_if(I, q0, ok0);
+      pair_add_if(I, q1, ok1);
+    }
+    if (e < f) {
+      uint j;
+      const bool ok = take(j);
+      pair_add_if(I, pair_term(P, I, srcA[S * (size_t)j], srcV[S * (size_t)j]), ok);
+    }
+    qn = max(qn - f, 0);
+  };
+  const int maxf = wave_max(nwords);
+  uint nx4[FW_AHEAD];
+#pragma unroll
+  for (int k = 0; k < FW_AHEAD; k++) nx4[k] = k < nwords ? bits[(size_t)k * NN + i] : 0u;
+  for (int f0 = 0; f0 < maxf; f0 += FW_AHEAD) {
+    uint cw[FW_AHEAD];
+#pragma unroll
+    for (int k = 0; k < FW_AHEAD; k++) {
+      cw[k] = nx4[k];
+      const int fn = f0 + FW_AHEAD + k;
+      nx4[k] = fn < nwords ? bits[(size_t)fn * NN + i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < FW_AHEAD; k++) {
+      const int f = f0 + k;
+      if (f >= maxf) break;
+      const uint word = cw[k];
+      if (word != 0u) {
+        while (f >= E_[0]) {  // move to the row holding word f
+#pragma unroll
+          for (int r = 0; r < R - 1; r++) {
+            E_[r] = E_[r + 1];
+            G[r] = G[r + 1];
+          }
+          E_[R - 1] = INT_MAX;
+        }
+      }
+      const int held = (cur != 0u ? 1 : 0) + (nxt != 0u ? 1 : 0) + qc;
+      const bool full = word != 0u && held == K + 2;
+      if (__any(full)) {  // free a word slot: the full lanes finish their current word
+        const int need = wave_max(full ? __popc(cur) : 0);
+        drain(max(need, wave_min(use ? qn : 0x7fffffff)));
+      }
+      if (word != 0u) {
+        const int base = G[0] + (f << 5);
+        if (cur == 0u) {
+          cur = word;
+          curb = base;
+        } else if (nxt == 0u) {
+          nxt = word;
+          nxtb = base;
+        } else {
+          int tl = qh + qc;
+          if (tl >= K) tl -= K;
+          Qw[tl * nt + t] = word;
+          Qb[tl * nt + t] = (uint)base;
+          qc++;
+        }
+        qn += __popc(word);
+      }
+    }
+  }
+  drain(wave_max(qn));
+}
+
+template <int SR>
+__device__ __forceinline__ void force_q_impl(const FParams_t& P, const Span S, int N, uint* lq) {
+  constexpr int R = SR * SR;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int i = S.p0 + S.blk * nt + t;
+  const uint* __restrict__ off = pbuf<const uint>(FAccel, AGRIDOFF);
+  const uint* __restrict__ cnt = pbuf<const uint>(FAccel, AGRIDCNT);
+  const uint* __restrict__ bits = pbuf<const uint>(FAccel, ANBRBITS);
+  const float4* __restrict__ fr = pbuf<const float4>(FAccel, AFREC);
+  int gc = FX_GRID_UNDEF;
+  if (i < S.p1) gc = pbuf<const int>(FPnts, FGCELL)[i];
+  const bool act = gc != FX_GRID_UNDEF;
+  const int shift = (P.gridRes.z + 1) * P.gridRes.x + 1;
+  ForceI I;
+  I.px = I.py = I.pz = I.vx = I.vy = I.vz = 0.0f;
+  I.di = 1.0f;
+  I.pi = 0.0f;
+  I.fx = I.fy = I.fz = 0.0f;
+  // the lane's row windows: row r covers words [W_r, W_{r+1}) of its bitmask;
+  // E_[r] = W_{r+1}, and the record of word f, bit b is G[r] + 32 f + b
+  int E_[R], G[R];
+  int nwords = 0;
+  if (act) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int c0 = gc + P.gridAdj[r * SR] - shift;
+      const int jb = (int)off[c0], je = (int)off[c0 + SR];
+      G[r] = jb - (nwords << 5);
+      nwords += (je - jb + 31) >> 5;
+      E_[r] = nwords;
+    }
+    const float4 a = fr[2 * (size_t)i], v = fr[2 * (size_t)i + 1];
+    I.px = a.x; I.py = a.y; I.pz = a.z;
+    I.vx = v.x; I.vy = v.y; I.vz = v.z;
+    I.di = a.w;
+    I.pi = (I.di - P.prest_dens) * P.pintstiff;
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; r++) E_[r] = G[r] = 0;
+  }
+  const bool ovf = act && nwords > FX_NBR_WORDS;
+  if (ovf) {  // no recorded bits: test every candidate
+    const int base = stencil_base(P, gc);
+    for (int r = 0; r < R; r++) {
+      int jb, je;
+      row_range(P, base, r, off, cnt, jb, je);
+      for (int j = jb; j < je; j++) {
+        const float4 a = fr[2 * (size_t)j];
+        if (is_pair(P, I, a)) force_pair(P, I, a, fr[2 * (size_t)j + 1]);
+      }
+    }
+  }
+  const bool use = act && !ovf;
+  const int K = FW_QBYTES / (8 * nt);
+  force_walk_q<R, 2>(P, I, bits, (size_t)N, i, use ? nwords : 0, E_, G, lq, lq + K * nt, K, fr, fr + 1, use);
+  if (act) pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{I.fx, I.fy, I.fz};
+}
+
+// Dispatch: the word queue needs the density stage's records and at least
+// 4 LDS words per lane (blocks of up to 256 threads); otherwise the ring or
+// candidate-testing pass.
+__device__ __forceinline__ void force_q(const FParams_t& P, const Span S, int N, uint* lq) {
+  if (FAccel.mbuf[ANBRBITS] == 0 || FAccel.mbuf[AFREC] == 0 || FW_QBYTES / (8 * (int)blockDim.x) < 4 ||
+      P.gridSrch > 3) {
+    force_impl(P, S, N, lq);
+    return;
+  }
+  if (P.gridSrch == 3)
+    force_q_impl<3>(P, S, N, lq);
+  else
+    force_q_impl<2>(P, S, N, lq);
+}
+
 extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
-  __shared__ uint fq[FQ_SLOTS];
-  force_impl(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, fq);
+  __shared__ uint lq[FW_QBYTES / 4];
+  force_q(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, lq);
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computeForceBatch(BatchArgs B, int ntotal) {
-  __shared__ uint fq[FQ_SLOTS];
+  __shared__ uint lq[FW_QBYTES / 4];
   const FParams_t* P;
   const Span S = batch_span(B, xcd_block(), P);
-  force_impl(*P, S, ntotal, fq);
+  force_q(*P, S, ntotal, lq);
 }
 
 // ------------------------------------------------------ neighbour statistics

@@ -164,8 +164,9 @@ def test_graph_equals_eager(ps):
 
 @pytest.mark.parametrize("threads", [64, 128, 512, 1024])
 def test_block_size_invariance(ps, threads):
-    """Results do not depend on the launch geometry (the force stage switches
-    from its pair ring to in-place evaluation above 64 threads per block)."""
+    """Results do not depend on the launch geometry (the force stage's word
+    queue holds 16, 8 or 4 LDS words per lane at 64, 128 and 256 threads per
+    block; larger blocks evaluate pairs in place)."""
     sc = scenes.dambreak_16k()
     defaults = [ps.StageBlockSize(k) for k in range(abi.NUM_STAGES)]
     ps.Restart(sc)
@@ -181,6 +182,29 @@ def test_block_size_invariance(ps, threads):
             ps.SetStageBlockSize(k, t)
     for k in ("pos", "vel", "veval", "press", "force"):
         assert bits_equal(base[k], g[k]), (threads, k)
+
+
+@pytest.mark.parametrize("gd,srch,threads", [(1.0, 2, 64), (1.0, 2, 256), (2.5, 4, 64), (2.5, 4, 256)])
+def test_stencil_widths(ps, gd, srch, threads):
+    """Neighbour searches of 2 and 4 cells per axis (4 and 16 stencil rows;
+    the force stage walks 2-cell rows with its word queue and 4-cell rows
+    with its pair ring), bit-exact with the oracle over 3 steps."""
+    sc = scenes.dambreak_16k()
+    sc.grid_density = gd
+    defaults = [ps.StageBlockSize(k) for k in range(abi.NUM_STAGES)]
+    ps.SetBlockSize(threads)
+    try:
+        ps.Restart(sc)
+        p = ps.params
+        assert p.gridSrch == srch
+        s0 = ps.state()
+        ps.Run(3)
+        g = ps.state()
+    finally:
+        for k, t in enumerate(defaults):
+            ps.SetStageBlockSize(k, t)
+    ref = oracle.run(p, fresh_oracle(p, s0), 3, threads=THREADS)
+    compare_state(g, ref)
 
 
 @pytest.mark.parametrize("n", [1, 2, 63, 65, 1000, 4097])
