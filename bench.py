@@ -80,7 +80,9 @@ def main():
                              "adjoint_batch32", "ref_beach"])
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="dambreak_4m: steps of the forward+adjoint rollout timed beside the plain step (0 = skip)")
-    ap.add_argument("--checkpoint-every", type=int, default=10)
+    ap.add_argument("--checkpoint-every", type=int, default=0,
+                    help="adjoint segment length; 0 = the fewest equal segments whose tape fits in 60%% of "
+                         "free device memory")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default block sizes instead of tuning them after the warm-up")
     ap.add_argument("--code-object", default=None, help="step code object to load (default: the built one)")
@@ -307,6 +309,7 @@ def adjoint_ratio(ps, restart, W, T, K, use_graph):
     tgt = np.empty((ps.NumPoints(), 3), np.float32)
     tgt[ids] = ps.Retrieve(abi.FPOS)
     ps.AdjointConfigure(T, K)
+    K = ps.AdjointSegment()
     ps.AdjointSetTarget(tgt)
     _, f_ms = ps.AdjointForward()
     g = ps.AdjointBackward()
@@ -336,7 +339,7 @@ def adjoint_bench(args):
     else:
         sc = scenes.batch32_scenes()
         T = args.steps if args.steps is not None else 40
-        K = args.checkpoint_every if args.checkpoint_every != 10 else 8
+        K = args.checkpoint_every
         # each scene fits its viscosity to the rollout at viscosity 0.35
         target = np.concatenate([target_positions(ps, s, T, visc=0.35, k=s.int_stiff) for s in sc])
         restart = lambda: ps.RestartBatch(sc)
@@ -358,6 +361,8 @@ def adjoint_bench(args):
         plain_ms, _ = ps.BenchSteps(T, use_graph=False)
     restart()
     ps.AdjointConfigure(T, K)
+    K = ps.AdjointSegment()
+    desc = desc.replace("checkpoints every 0", f"checkpoints every {K}")
     ps.AdjointSetTarget(target)
     loss, f_ms = ps.AdjointForward()
     g = ps.AdjointBackward()

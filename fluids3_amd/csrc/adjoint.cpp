@@ -79,7 +79,7 @@ void Particles::AdjRelease() {
 
 int Particles::AdjConfigure(const char* path, int nsteps, int every) {
   if (mNumPoints <= 0) return fx::fail(FX_ESTATE, "adjoint: call Restart first");
-  if (nsteps < 0 || every < 1) return fx::fail(FX_EINVAL, "adjoint: nsteps %d, every %d", nsteps, every);
+  if (nsteps < 0) return fx::fail(FX_EINVAL, "adjoint: nsteps %d", nsteps);
   if (m_AdjModule == nullptr) {
     std::string p;
     if (path != nullptr) {
@@ -94,9 +94,29 @@ int Particles::AdjConfigure(const char* path, int nsteps, int every) {
   FX_HIP(hipStreamSynchronize(m_Stream), "adjoint");
   AdjRelease();
   const int n = mNumPoints;
+  const size_t tape_step = kTapeBytes * (size_t)n + 8 * (size_t)m_ScanCells;
+  if (every <= 0) {
+    // The backward recomputes every segment but the last, (segments - 1) x
+    // every steps: take the fewest segments whose tape (with its bitmasks)
+    // and checkpoints fit in 60 % of the free device memory, of equal length.
+    size_t free_b = 0, total_b = 0;
+    FX_HIP(hipMemGetInfo(&free_b, &total_b), "adjoint: querying device memory");
+    const double budget = 0.6 * (double)free_b;
+    const double fixed = (double)(sizeof(float) * 22 + 48 + 16) * n;
+    const double per_step = (double)tape_step + 4.0 * FX_NBR_WORDS * n;
+    int longest = 1;  // longest segment that fits
+    for (int e = nsteps > 1 ? nsteps : 1; e >= 1; e--) {
+      const int sg = (nsteps + e - 1) / e;
+      if (fixed + (double)kCkptBytes * n * (sg + 1) + per_step * (nsteps < e ? nsteps : e) <= budget) {
+        longest = e;
+        break;
+      }
+    }
+    const int sg = nsteps > 0 ? (nsteps + longest - 1) / longest : 1;
+    every = nsteps > 0 ? (nsteps + sg - 1) / sg : 1;
+  }
   const int segs = nsteps == 0 ? 0 : (nsteps + every - 1) / every;
   const int tape_steps = nsteps < every ? nsteps : every;
-  const size_t tape_step = kTapeBytes * (size_t)n + 8 * (size_t)m_ScanCells;
   const int scenes = NumScenes();
   int blocks = 0;
   for (int s = 0; s < scenes; s++) blocks += (m_PBase[s + 1] - m_PBase[s] + kAdjThreads - 1) / kAdjThreads;

@@ -31,6 +31,8 @@
 // trees.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "fluidsx_params.h"
 
 typedef unsigned int uint;
@@ -386,6 +388,131 @@ __device__ __forceinline__ void ring_walk(const FParams_t& P, const AdjArgs& A, 
   drain(min(wave_max_i(qn), fqn));
 }
 
+// Walks the lane's recorded pairs through a word queue: each lane reads its
+// bitmask words as one stream (row r's words follow row r-1's) and queues
+// every non-zero word whole, with the record index of its bit 0 -- two words
+// in registers, up to AW_QWORDS more in LDS; the wave then takes the same
+// number of pairs from every lane (as many as the shortest queue holds, more
+// only when a queue must make room), two per step, each lane's in candidate
+// order.  Same contract as ring_walk; for stencil rows of SR cells, K LDS
+// words per lane (8 K bytes per block).
+#ifndef AW_QF
+#define AW_QF 8   // LDS words per lane in the force adjoint
+#endif
+#ifndef AW_QD
+#define AW_QD 8   // and in the density adjoint
+#endif
+#ifndef AW_FORCE_ATTR
+#define AW_FORCE_ATTR
+#endif
+
+template <int SR, int K, class Op>
+__device__ __forceinline__ void queue_walk(const FParams_t& P, const AdjArgs& A, int i, int base, bool use,
+                                           uint* lq, Op& op) {
+  constexpr int R = SR * SR;
+  const int t = threadIdx.x;
+  const size_t NN = (size_t)A.nbits;
+  // row r covers words [W_r, W_{r+1}) of the lane's bitmask; E_[r] = W_{r+1},
+  // and the record of word f, bit b is G[r] + 32 f + b
+  int E_[R], G[R];
+  int nwords = 0;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    int jb = 0, je = 0;
+    if (use) row_of(P, base, r, A.off, A.cnt, jb, je);
+    G[r] = jb - (nwords << 5);
+    nwords += (je - jb + 31) >> 5;
+    E_[r] = nwords;
+  }
+  uint* Qw = lq;
+  uint* Qb = lq + K * 64;
+  uint cur = 0u, nxt = 0u;
+  int curb = 0, nxtb = 0, qh = 0, qc = 0, qn = 0;
+  auto take = [&](uint& j) -> bool {
+    const bool ok = cur != 0u;
+    j = ok ? (uint)(curb + __builtin_ctz(cur)) : 0u;
+    cur &= cur - 1u;
+    if (ok && cur == 0u) {
+      cur = nxt;
+      curb = nxtb;
+      nxt = 0u;
+      if (qc > 0) {
+        nxt = Qw[qh * 64 + t];
+        nxtb = (int)Qb[qh * 64 + t];
+        qh = (qh + 1 == K) ? 0 : qh + 1;
+        qc--;
+      }
+    }
+    return ok;
+  };
+  auto drain = [&](int f) {
+    int e = 0;
+    for (; e + 2 <= f; e += 2) {
+      uint j0, j1;
+      const bool ok0 = take(j0);
+      const bool ok1 = take(j1);
+      op.eval(j0, ok0);
+      op.eval(j1, ok1);
+    }
+    if (e < f) {
+      uint j;
+      const bool ok = take(j);
+      op.eval(j, ok);
+    }
+    qn = max(qn - f, 0);
+  };
+  const int maxf = wave_max_i(nwords);
+  for (int f = 0; f < maxf; f++) {
+    const uint word = f < nwords ? A.bits[(size_t)f * NN + i] : 0u;
+    if (word != 0u) {
+      while (f >= E_[0]) {  // move to the row holding word f
+#pragma unroll
+        for (int r = 0; r < R - 1; r++) {
+          E_[r] = E_[r + 1];
+          G[r] = G[r + 1];
+        }
+        E_[R - 1] = INT_MAX;
+      }
+    }
+    const int held = (cur != 0u ? 1 : 0) + (nxt != 0u ? 1 : 0) + qc;
+    const bool full = word != 0u && held == K + 2;
+    if (__any(full)) {  // free a word slot: the full lanes finish their current word
+      const int need = wave_max_i(full ? __popc(cur) : 0);
+      drain(max(need, wave_min_i(use ? qn : 0x7fffffff)));
+    }
+    if (word != 0u) {
+      const int wb = G[0] + (f << 5);
+      if (cur == 0u) {
+        cur = word;
+        curb = wb;
+      } else if (nxt == 0u) {
+        nxt = word;
+        nxtb = wb;
+      } else {
+        int tl = qh + qc;
+        if (tl >= K) tl -= K;
+        Qw[tl * 64 + t] = word;
+        Qb[tl * 64 + t] = (uint)wb;
+        qc++;
+      }
+      qn += __popc(word);
+    }
+  }
+  drain(wave_max_i(qn));
+}
+
+// Stencil rows of 2 or 3 cells through the word queue, 4 through the ring.
+template <int K, class Op>
+__device__ __forceinline__ void pair_walk(const FParams_t& P, const AdjArgs& A, int i, int base, bool use,
+                                          uint* lq, Op& op) {
+  if (P.gridSrch == 3)
+    queue_walk<3, K>(P, A, i, base, use, lq, op);
+  else if (P.gridSrch == 2)
+    queue_walk<2, K>(P, A, i, base, use, lq, op);
+  else
+    ring_walk(P, A, i, base, use, lq, op);
+}
+
 // ------------------------------------------------------------------ force
 
 // Pair term of the force adjoint; same expression tree as the CPU oracle.
@@ -437,8 +564,8 @@ struct ForceAdj {
   }
 };
 
-extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjForce(AdjArgs A) {
-  __shared__ uint fq[AQ_SLOTS];
+extern "C" __global__ void __launch_bounds__(ADJ_THREADS) AW_FORCE_ATTR adjForce(AdjArgs A) {
+  __shared__ uint fq[(2 * AW_QF * 64 > AQ_SLOTS) ? 2 * AW_QF * 64 : AQ_SLOTS];
   const Slot S = slot_of(A);
   const FParams_t& P = *S.P;
   const int i = S.i;
@@ -484,7 +611,7 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjForce(AdjArgs A) {
       }
     }
   }
-  ring_walk(P, A, i, base, act && !ovf, fq, F);
+  pair_walk<AW_QF>(P, A, i, base, act && !ovf, fq, F);
   if (act) {
     const fx_float3 x0 = A.gxs[i], e0 = A.ges[i];
     A.gxs[i] = fx_float3{x0.x + F.gpx, x0.y + F.gpy, x0.z + F.gpz};
@@ -524,7 +651,7 @@ struct DensityAdj {
 };
 
 extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjDensity(AdjArgs A) {
-  __shared__ uint fq[AQ_SLOTS];
+  __shared__ uint fq[(2 * AW_QD * 64 > AQ_SLOTS) ? 2 * AW_QD * 64 : AQ_SLOTS];
   const Slot S = slot_of(A);
   const FParams_t& P = *S.P;
   const int i = S.i;
@@ -556,7 +683,7 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjDensity(AdjArgs A) 
       }
     }
   }
-  ring_walk(P, A, i, base, act && !ovf, fq, D);
+  pair_walk<AW_QD>(P, A, i, base, act && !ovf, fq, D);
   if (i >= S.p1) return;
   const uint id = A.id[i];
   const fx_float3 x0 = A.gxs[i];

@@ -233,6 +233,12 @@ int fx_adjoint_backward(fx_system* s, float* gpos, float* gvel, float* gveval, d
 
 int64_t fx_adjoint_memory(const fx_system* s) { return s == nullptr ? 0 : s->p.AdjMemory(); }
 
+int fx_adjoint_segment(const fx_system* s, int32_t* every) {
+  if (s == nullptr || every == nullptr) return fx::fail(FX_EINVAL, "null argument");
+  *every = s->p.AdjSegment();
+  return FX_OK;
+}
+
 const char* fx_last_error(void) { return fx::last_error().c_str(); }
 
 const char* fx_version(void) { return "fluidsx 0.1 gfx950"; }

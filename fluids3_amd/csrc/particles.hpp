@@ -116,6 +116,7 @@ class Particles {
   int AdjForward(double* loss, double* ms);
   int AdjBackward(float* gpos, float* gvel, float* gveval, double* gvisc, double* gk, double* ms);
   int64_t AdjMemory() const { return m_AdjBytes; }
+  int AdjSegment() const { return m_AdjEvery; }
 
   int NumPoints() const { return mNumPoints; }
   int CellTableSize() const { return m_ScanCells; }

@@ -64,6 +64,7 @@ SIGNATURES = {
     "fx_adjoint_backward": (C.c_int, [_P, _P, _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]),
     "fx_adjoint_memory": (C.c_int64, [_P]),
+    "fx_adjoint_segment": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "fx_last_error": (C.c_char_p, []),
     "fx_version": (C.c_char_p, []),
 }
@@ -315,10 +316,17 @@ class Particles:
         return {abi.STAGE_NAMES[k]: (th[k], ms[k]) for k in range(abi.NUM_STAGES)}
 
     # ---- reverse mode
-    def AdjointConfigure(self, nsteps, every, code_object=None):
-        """Checkpoints every `every` steps of an nsteps rollout from the current state."""
+    def AdjointConfigure(self, nsteps, every=0, code_object=None):
+        """Checkpoints every `every` steps of an nsteps rollout from the current state
+        (every <= 0: the fewest equal segments that fit in 60 % of the free device memory)."""
         path = None if code_object is None else code_object.encode()
         check(lib().fx_adjoint_configure(self._h, path, nsteps, every), "fx_adjoint_configure")
+
+    def AdjointSegment(self):
+        """Segment length in use (the chosen one after AdjointConfigure(n, 0))."""
+        e = C.c_int32()
+        check(lib().fx_adjoint_segment(self._h, C.byref(e)), "fx_adjoint_segment")
+        return e.value
 
     def AdjointSetTarget(self, target):
         """Target positions, (n, 3) float32 in particle-id order."""

@@ -214,13 +214,19 @@ int fx_autotune(fx_system* sys, int32_t nsteps, int32_t* threads_out, double* ms
  * stiffness.  Adjoint kernels of density, force and advance run over the
  * recorded forward steps; neighbour sets, cells, wall and limiter branches
  * and the sort permutation of the forward are held fixed.  State is
- * checkpointed every `checkpoint_every` steps and each segment is recomputed
- * once in the backward pass. */
+ * checkpointed every `checkpoint_every` steps; the forward records the last
+ * segment's steps and the backward recomputes each other segment once. */
 
 /* Loads the adjoint code object (NULL: adjoint_gfx950.hsaco beside the step's
- * code object) and allocates checkpoints and tape for the current Restart. */
+ * code object) and allocates checkpoints and tape for the current Restart.
+ * checkpoint_every <= 0 chooses the fewest segments, of equal length, whose
+ * tape and checkpoints fit in 60 % of the device memory free at the call (the
+ * backward recomputes all segments but the last; a rollout in one segment
+ * needs no recomputation). */
 int fx_adjoint_configure(fx_system* sys, const char* code_object_path, int32_t nsteps,
                          int32_t checkpoint_every);
+/* The segment length in use (the chosen one when configured with <= 0). */
+int fx_adjoint_segment(const fx_system* sys, int32_t* checkpoint_every);
 /* Target positions x*, 3 floats per particle in particle-id order (count =
  * particles).  Defaults to zero. */
 int fx_adjoint_set_target(fx_system* sys, const float* target, int64_t count);

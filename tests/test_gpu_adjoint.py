@@ -56,6 +56,25 @@ def test_adjoint_bitexact_against_fp32_oracle(ps):
     assert g["gk"][0] == pytest.approx(r["gk"], rel=1e-9)
 
 
+@pytest.mark.parametrize("gd,srch", [(1.0, 2), (2.5, 4)])
+def test_adjoint_stencil_widths(ps, gd, srch):
+    """Searches of 2 and 4 cells per axis (the pair kernels' word queue and
+    their pair ring), bit-exact with the FP32 oracle adjoint."""
+    sc = scenes.scaled_dam_break(3000)
+    sc.grid_density = gd
+    T = 4
+    P = derive_params(sc)
+    assert P.gridSrch == srch
+    target = target_positions(ps, sc, T)
+    g = gpu_grad(ps, sc, T, 3, target)
+    pos, clr = fill_points(sc)
+    r = oracle.grad(P, {"pos": pos, "clr": clr}, T, target, fp64=False, threads=THREADS)
+    for k in ("gpos", "gvel", "gveval"):
+        assert bits_equal(g[k], r[k]), (k, np.abs(g[k] - r[k]).max())
+    assert g["gvisc"][0] == pytest.approx(r["gvisc"], rel=1e-9)
+    assert g["gk"][0] == pytest.approx(r["gk"], rel=1e-9)
+
+
 def test_adjoint_dense_block_without_bitmask(ps):
     # ~180 particles per cell: interior particles need more than FX_NBR_WORDS
     # bitmask words, so their pairs are found by testing candidates in place
@@ -111,8 +130,9 @@ def test_adjoint_checkpoint_spacing_invariance(ps):
     T = 9
     target = target_positions(ps, sc, T)
     ref = gpu_grad(ps, sc, T, T, target)
-    for every in (1, 2, 4):
+    for every in (1, 2, 4, 0):  # 0: chosen from free memory (all 9 steps fit one segment)
         g = gpu_grad(ps, sc, T, every, target)
+        assert ps.AdjointSegment() == (every if every > 0 else T)
         for k in ("gpos", "gvel", "gveval"):
             assert bits_equal(g[k], ref[k]), (every, k)
         for k in ("loss", "gvisc", "gk"):
