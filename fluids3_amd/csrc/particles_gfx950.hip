@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <type_traits>
 
 #include "fluidsx_params.h"
 
@@ -458,6 +459,49 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
     if (act) {
       jb = (int)off[gc + adj];
       je = (int)(off[gc + adj + srch - 1] + cnt[gc + adj + srch - 1]);
+    }
+    if (hi - lo <= TILE_P) {
+      // The row's run fits one tile: the lane's candidate s (0 <= s < len)
+      // is tile entry a + s and bit s % 32 of the row's word s / 32, the
+      // same position in every lane.  The wave walks s in lockstep, 32
+      // candidates per word; candidates every lane has are unchecked, the
+      // rest are masked by s < len.  Bits are shifted in (first candidate
+      // highest) and the word is bit-reversed when stored.
+      for (int k = t; k < hi - lo; k += blockDim.x) {
+        const fx_float3 q = pos[lo + k];
+        tile[k] = make_float4(q.x, q.y, q.z, 0.0f);
+      }
+      __syncthreads();
+      const int a = act ? jb - lo : 0, len = act ? je - jb : 0;
+      const int lmax = wave_max(len), lmin = wave_min(act ? len : INT_MAX);
+      const int last = hi - lo - 1;  // last entry of the run
+      const int nw = (len + 31) >> 5;
+      uint acc = 0u;
+      auto cand = [&](int s, auto checked) {
+        int k = a + s;
+        if (decltype(checked)::value) k = min(k, last);
+        const float4 q = tile[k];
+        const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        bool pair = dsq < rd2 && dsq > 0.0f;
+        if (decltype(checked)::value) pair = pair && s < len;
+        const float tt = pair ? (rd2 - dsq) * d2 : 0.0f;
+        sum = fmaf(tt * tt, tt, sum);  // + 0 when not a pair: sum unchanged
+        acc = acc + acc + (pair ? 1u : 0u);
+      };
+      for (int s0 = 0; s0 < lmax; s0 += 32) {
+        const int n = min(32, lmax - s0);
+        const int nu = max(0, min(n, lmin - s0));
+        int u = 0;
+#pragma unroll 4
+        for (; u < nu; u++) cand(s0 + u, std::false_type());
+        for (; u < n; u++) cand(s0 + u, std::true_type());
+        if (rec && (s0 >> 5) < nw) bits[(size_t)(W + (s0 >> 5)) * NN + i] = __builtin_bitreverse32(acc << (32 - n));
+        acc = 0u;
+      }
+      __syncthreads();
+      W += (je - jb + 31) >> 5;
+      continue;
     }
     uint acc = 0, bit = 1u;  // bits of the current word, and the next one's mask
     int wcur = W;            // current word
