@@ -80,9 +80,10 @@ def main():
                              "adjoint_batch32", "ref_beach"])
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="dambreak_4m: steps of the forward+adjoint rollout timed beside the plain step (0 = skip)")
-    ap.add_argument("--checkpoint-every", type=int, default=0,
-                    help="adjoint segment length; 0 = the fewest equal segments whose tape fits in 60%% of "
-                         "free device memory")
+    ap.add_argument("--checkpoint-every", type=int, default=None,
+                    help="adjoint segment length (default: 10 for adjoint_262k, 8 for adjoint_batch32, as those "
+                         "workloads specify; 0 for the 4M adjoint ratio); 0 = the fewest equal segments whose "
+                         "tape fits in 60%% of free device memory")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default block sizes instead of tuning them after the warm-up")
     ap.add_argument("--code-object", default=None, help="step code object to load (default: the built one)")
@@ -244,7 +245,8 @@ def main():
     # forward+adjoint rollout beside the plain step, same start state and window
     adjoint = None
     if args.workload == "dambreak_4m" and args.adjoint_steps > 0:
-        adjoint = adjoint_ratio(ps, restart, W, args.adjoint_steps, args.checkpoint_every, use_graph)
+        K = args.checkpoint_every if args.checkpoint_every is not None else 0
+        adjoint = adjoint_ratio(ps, restart, W, args.adjoint_steps, K, use_graph)
 
     cpu = None
     if want_cpu:
@@ -331,7 +333,7 @@ def adjoint_bench(args):
     if args.workload == "adjoint_262k":
         sc = scenes.dambreak_262k()
         T = args.steps if args.steps is not None else 50
-        K = args.checkpoint_every
+        K = args.checkpoint_every if args.checkpoint_every is not None else 10
         target = target_positions(ps, sc, T)
         restart = lambda: ps.Restart(sc)
         desc = (f"adjoint_262k: differentiable dam break, {sc.num_points:,} particles, {T} steps, checkpoints every {K}, "
@@ -339,7 +341,7 @@ def adjoint_bench(args):
     else:
         sc = scenes.batch32_scenes()
         T = args.steps if args.steps is not None else 40
-        K = args.checkpoint_every
+        K = args.checkpoint_every if args.checkpoint_every is not None else 8
         # each scene fits its viscosity to the rollout at viscosity 0.35
         target = np.concatenate([target_positions(ps, s, T, visc=0.35, k=s.int_stiff) for s in sc])
         restart = lambda: ps.RestartBatch(sc)
@@ -359,14 +361,25 @@ def adjoint_bench(args):
         print(f"graph-timed region unavailable ({e}); timing eager launches", file=sys.stderr)
         restart()
         plain_ms, _ = ps.BenchSteps(T, use_graph=False)
-    restart()
-    ps.AdjointConfigure(T, K)
-    K = ps.AdjointSegment()
+
+    def rollout(k):
+        restart()
+        ps.AdjointConfigure(T, k)
+        k = ps.AdjointSegment()
+        ps.AdjointSetTarget(target)
+        loss, f_ms = ps.AdjointForward()
+        g = ps.AdjointBackward()
+        g["bytes"] = ps.AdjointMemory()
+        return k, loss, f_ms, g
+
+    K, loss, f_ms, g = rollout(K)
     desc = desc.replace("checkpoints every 0", f"checkpoints every {K}")
-    ps.AdjointSetTarget(target)
-    loss, f_ms = ps.AdjointForward()
-    g = ps.AdjointBackward()
     tot = f_ms + g["ms"]
+    # the same rollout with segments chosen from free device memory, reported beside it
+    ka, _, fa, ga = rollout(0)
+    auto = {"checkpoint_every": ka, "forward_ms": round(fa, 4), "backward_ms": round(ga["ms"], 4),
+            "fwd_adjoint_over_plain": round((fa + ga["ms"]) / plain_ms, 3),
+            "tape_and_checkpoint_bytes": ga["bytes"]}
     out = {
         "metric": "forward+adjoint particle-steps/sec",
         "value": round(N * T / (tot * 1e-3), 1),
@@ -383,9 +396,10 @@ def adjoint_bench(args):
         "config": {"workload": desc, "particles": N, "checkpoint_every": K,
                    "plain_ms": round(plain_ms, 4), "forward_ms": round(f_ms, 4), "backward_ms": round(g["ms"], 4),
                    "fwd_adjoint_over_plain": round(tot / plain_ms, 3),
-                   "tape_and_checkpoint_bytes": ps.AdjointMemory(),
+                   "tape_and_checkpoint_bytes": g["bytes"],
                    "loss": [round(float(x), 6) for x in loss[:8]],
-                   "dL_dvisc": [float(x) for x in g["gvisc"][:8]], "dL_dstiff": [float(x) for x in g["gk"][:8]]},
+                   "dL_dvisc": [float(x) for x in g["gvisc"][:8]], "dL_dstiff": [float(x) for x in g["gk"][:8]],
+                   "memory_chosen_segments": auto},
         "roofline": None,
         "cpu_baseline": None,
     }

@@ -9,9 +9,15 @@
 // the tape is recomputed from its checkpoint (the step is deterministic, so
 // the recomputed steps are bit-identical to the forward's), recording each
 // step right after its force stage; then the adjoint kernels run over the
-// recorded steps in reverse.  Memory: (segments + 1) x 44 B x N of
-// checkpoints plus every x (60 B x N + 8 B x cells + FX_NBR_WORDS x 4 B x N
-// of neighbour bitmask) of tape.
+// recorded steps in reverse.  The adjoint kernels run on a second stream, so
+// that segment s-1 is recomputed (into a second tape buffer) while the
+// adjoint of segment s runs.  For the length of a rollout the particles are
+// numbered by their slot at its start (see adjFrameIn).  Memory: (segments
+// + 1) x 44 B x N of checkpoints plus every x (60 B x N + 8 B x cells +
+// FX_NBR_WORDS x 4 B x N of neighbour bitmask) of tape, twice when there is
+// more than one segment and the second buffer fits in 60 % of the free
+// device memory.
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -20,8 +26,9 @@
 
 namespace {
 
-enum { ADJ_LOSS, ADJ_ADVANCE, ADJ_FORCE, ADJ_DENSITY, ADJ_REDUCE };
-const char* kAdjNames[5] = {"adjLoss", "adjAdvance", "adjForce", "adjDensity", "adjReduce"};
+enum { ADJ_LOSS, ADJ_ADVANCE, ADJ_FORCE, ADJ_DENSITY, ADJ_REDUCE, ADJ_FRAME_IN, ADJ_FRAME_OUT, ADJ_UNFRAME, ADJ_NFUNC };
+const char* kAdjNames[ADJ_NFUNC] = {"adjLoss",    "adjAdvance",  "adjForce",  "adjDensity",
+                                    "adjReduce",  "adjFrameIn",  "adjFrameOut", "adjUnframe"};
 const int kAdjThreads = 64;   // ADJ_THREADS: one wave per block
 const int kRedThreads = 256;  // RED_THREADS
 
@@ -61,6 +68,13 @@ const size_t kTapeBytes = 60;   // pos, vel, veval, force (12 B each), rho, id, 
 // plus, per recorded step, the density kernel's neighbour bitmask
 // (FX_NBR_WORDS words per particle), written there directly by the forward
 
+// one thread per particle, 256 per block
+int launch_n(hipFunction_t f, const char* name, hipStream_t st, int n, void** args) {
+  if (n <= 0) return FX_OK;
+  FX_HIP(hipModuleLaunchKernel(f, (unsigned)((n + 255) / 256), 1, 1, 256, 1, 1, 0, st, args, nullptr), name);
+  return FX_OK;
+}
+
 }  // namespace
 
 void Particles::AdjRelease() {
@@ -70,6 +84,7 @@ void Particles::AdjRelease() {
   m_AdjBits = nullptr;
   m_AdjArec = m_AdjDrec = m_AdjTabs = nullptr;
   m_AdjVec = nullptr;
+  m_AdjId0 = nullptr;
   m_AdjPartial = m_AdjAccum = nullptr;
   m_AdjParams = m_AdjPBase = m_AdjBTab = m_AdjBFirst = nullptr;
   m_AdjBytes = 0;
@@ -89,21 +104,27 @@ int Particles::AdjConfigure(const char* path, int nsteps, int every) {
       p = (k == std::string::npos ? std::string() : m_CodePath.substr(0, k + 1)) + "adjoint_gfx950.hsaco";
     }
     FX_HIP(hipModuleLoad(&m_AdjModule, p.c_str()), "adjoint: loading the adjoint code object");
-    for (int f = 0; f < 5; f++) FX_HIP(hipModuleGetFunction(&m_AdjFunc[f], m_AdjModule, kAdjNames[f]), kAdjNames[f]);
+    for (int f = 0; f < ADJ_NFUNC; f++)
+      FX_HIP(hipModuleGetFunction(&m_AdjFunc[f], m_AdjModule, kAdjNames[f]), kAdjNames[f]);
+  }
+  if (m_AdjStream == nullptr) {
+    FX_HIP(hipStreamCreate(&m_AdjStream), "adjoint: stream");
+    for (hipEvent_t& e : m_AdjEv) FX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), "adjoint: events");
   }
   FX_HIP(hipStreamSynchronize(m_Stream), "adjoint");
   AdjRelease();
   const int n = mNumPoints;
   const size_t tape_step = kTapeBytes * (size_t)n + 8 * (size_t)m_ScanCells;
+  // memory budget: 60 % of the free device memory
+  size_t free_b = 0, total_b = 0;
+  FX_HIP(hipMemGetInfo(&free_b, &total_b), "adjoint: querying device memory");
+  const double budget = 0.6 * (double)free_b;
+  const double fixed = (double)(sizeof(float) * 25 + 4 + 48 + 16) * n;
+  const double per_step = (double)tape_step + 4.0 * FX_NBR_WORDS * n;
   if (every <= 0) {
     // The backward recomputes every segment but the last, (segments - 1) x
     // every steps: take the fewest segments whose tape (with its bitmasks)
-    // and checkpoints fit in 60 % of the free device memory, of equal length.
-    size_t free_b = 0, total_b = 0;
-    FX_HIP(hipMemGetInfo(&free_b, &total_b), "adjoint: querying device memory");
-    const double budget = 0.6 * (double)free_b;
-    const double fixed = (double)(sizeof(float) * 22 + 48 + 16) * n;
-    const double per_step = (double)tape_step + 4.0 * FX_NBR_WORDS * n;
+    // and checkpoints fit the budget, of equal length.
     int longest = 1;  // longest segment that fits
     for (int e = nsteps > 1 ? nsteps : 1; e >= 1; e--) {
       const int sg = (nsteps + e - 1) / e;
@@ -116,7 +137,16 @@ int Particles::AdjConfigure(const char* path, int nsteps, int every) {
     every = nsteps > 0 ? (nsteps + sg - 1) / sg : 1;
   }
   const int segs = nsteps == 0 ? 0 : (nsteps + every - 1) / every;
-  const int tape_steps = nsteps < every ? nsteps : every;
+  const int seg_steps = nsteps < every ? nsteps : every;
+  // two tape buffers when segments are recomputed and the second one fits
+  // the budget: the backward then recomputes segment s-1 into one while the
+  // adjoint of segment s reads the other
+  // (FX_ADJOINT_TAPE_BUFFERS=1 in the environment keeps one buffer)
+  const char* one = getenv("FX_ADJOINT_TAPE_BUFFERS");
+  const bool two = segs > 1 && fixed + (double)kCkptBytes * n * (segs + 1) + 2.0 * per_step * seg_steps <= budget &&
+                   !(one != nullptr && one[0] == '1');
+  const int bufs = two ? 2 : 1;
+  const int tape_steps = bufs * seg_steps;
   const int scenes = NumScenes();
   int blocks = 0;
   for (int s = 0; s < scenes; s++) blocks += (m_PBase[s + 1] - m_PBase[s] + kAdjThreads - 1) / kAdjThreads;
@@ -132,8 +162,10 @@ int Particles::AdjConfigure(const char* path, int nsteps, int every) {
   m_AdjCkpt = static_cast<char*>(p);
   FX_TRY(alloc(tape_step * (size_t)tape_steps, &p));
   m_AdjTape = static_cast<char*>(p);
-  FX_TRY(alloc(sizeof(float) * (size_t)n * 22, &p));
+  FX_TRY(alloc(sizeof(float) * (size_t)n * 25, &p));
   m_AdjVec = static_cast<float*>(p);
+  FX_TRY(alloc(sizeof(uint32_t) * (size_t)n, &p));
+  m_AdjId0 = static_cast<uint32_t*>(p);
   FX_TRY(alloc(sizeof(uint32_t) * FX_NBR_WORDS * (size_t)n * (size_t)tape_steps, &p));
   m_AdjBits = static_cast<char*>(p);
   FX_TRY(alloc(48 * (size_t)n, &m_AdjArec));
@@ -182,6 +214,8 @@ int Particles::AdjConfigure(const char* path, int nsteps, int every) {
   m_AdjScenes = scenes;
   m_AdjCkptTime.assign(segs + 1, 0.0f);
   m_AdjTapeTime.assign(tape_steps, 0.0f);
+  m_AdjBufs = bufs;
+  m_AdjTapeSteps = seg_steps;
   return FX_OK;
 }
 
@@ -207,11 +241,10 @@ int Particles::AdjSetTarget(const float* target, int64_t count) {
   return FX_OK;
 }
 
-int Particles::AdjLaunch(int fid, void* args) {
+int Particles::AdjLaunch(int fid, void* args, hipStream_t st) {
   if (m_AdjBlocks == 0) return FX_OK;
   void* a[] = {args};
-  FX_HIP(hipModuleLaunchKernel(m_AdjFunc[fid], (unsigned)m_AdjBlocks, 1, 1, kAdjThreads, 1, 1, 0, m_Stream, a,
-                               nullptr),
+  FX_HIP(hipModuleLaunchKernel(m_AdjFunc[fid], (unsigned)m_AdjBlocks, 1, 1, kAdjThreads, 1, 1, 0, st, a, nullptr),
          kAdjNames[fid]);
   return FX_OK;
 }
@@ -284,11 +317,19 @@ int Particles::AdjForward(double* loss, double* ms) {
     FX_HIP(e1.create(), "adjoint");
     FX_HIP(hipEventRecord(e0.e, m_Stream), "adjoint");
   }
+  {  // the rollout frame: FID := slot, the target follows
+    void* fid = m_Points.gpu(FID);
+    float* tgt = m_AdjVec + 19 * (size_t)m_AdjN;
+    float* tgf = m_AdjVec + 22 * (size_t)m_AdjN;
+    int nn = m_AdjN;
+    void* a[] = {&fid, &m_AdjId0, &tgt, &tgf, &nn};
+    FX_TRY(launch_n(m_AdjFunc[ADJ_FRAME_IN], kAdjNames[ADJ_FRAME_IN], m_Stream, nn, a));
+  }
   const int last = m_AdjSegs - 1;
   for (int t = 0; t < m_AdjSteps; t++) {
     const int seg = t / m_AdjEvery;
     if (t % m_AdjEvery == 0) FX_TRY(AdjSaveCheckpoint(seg));
-    FX_TRY(AdjStep(seg == last ? t - seg * m_AdjEvery : -1));
+    FX_TRY(AdjStep(seg == last ? (seg % m_AdjBufs) * m_AdjTapeSteps + t - seg * m_AdjEvery : -1));
   }
   FX_TRY(AdjSaveCheckpoint(m_AdjSegs));
   // loss and seed on the final state
@@ -300,17 +341,23 @@ int Particles::AdjForward(double* loss, double* ms) {
   A.btab = m_AdjBTab;
   A.pos = m_Points.gpu(FPOS);
   A.id = m_Points.gpu(FID);
-  A.target = V + 19 * n;
+  A.target = V + 22 * n;
   A.gx = V;
   A.gv = V + 3 * n;
   A.ge = V + 6 * n;
   A.partial = m_AdjPartial;
-  FX_TRY(AdjLaunch(ADJ_LOSS, &A));
+  FX_TRY(AdjLaunch(ADJ_LOSS, &A, m_Stream));
   int k0 = 0, k1 = 1, add = 0;
   void* rargs[] = {&m_AdjPartial, &m_AdjBFirst, &m_AdjAccum, &k0, &k1, &add};
   FX_HIP(hipModuleLaunchKernel(m_AdjFunc[ADJ_REDUCE], (unsigned)m_AdjScenes, 1, 1, kRedThreads, 1, 1, 0, m_Stream,
                                rargs, nullptr),
          "adjReduce");
+  {  // the system leaves the call with its original ids
+    void* fid = m_Points.gpu(FID);
+    int nn = m_AdjN;
+    void* a[] = {&fid, &m_AdjId0, &nn};
+    FX_TRY(launch_n(m_AdjFunc[ADJ_FRAME_OUT], kAdjNames[ADJ_FRAME_OUT], m_Stream, nn, a));
+  }
   if (ms) FX_HIP(hipEventRecord(e1.e, m_Stream), "adjoint");
   std::vector<double> acc(3 * (size_t)m_AdjScenes);
   FX_HIP(hipMemcpyAsync(acc.data(), m_AdjAccum, acc.size() * sizeof(double), hipMemcpyDeviceToHost, m_Stream),
@@ -330,6 +377,20 @@ int Particles::AdjForward(double* loss, double* ms) {
   return FX_OK;
 }
 
+// Segment seg recomputed from its checkpoint on the system stream and
+// recorded into tape buffer seg % m_AdjBufs, once the adjoint that last read
+// that buffer has finished.
+int Particles::AdjRecompute(int seg) {
+  const int b = seg % m_AdjBufs;
+  FX_HIP(hipStreamWaitEvent(m_Stream, m_AdjEv[2 + b], 0), "adjoint");
+  FX_TRY(AdjRestoreCheckpoint(seg));
+  const int s0 = seg * m_AdjEvery;
+  const int s1 = (s0 + m_AdjEvery < m_AdjSteps) ? s0 + m_AdjEvery : m_AdjSteps;
+  for (int t = s0; t < s1; t++) FX_TRY(AdjStep(b * m_AdjTapeSteps + t - s0));
+  FX_HIP(hipEventRecord(m_AdjEv[b], m_Stream), "adjoint");
+  return FX_OK;
+}
+
 int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvisc, double* gk, double* ms) {
   if (!m_AdjLossDone) return fx::fail(FX_ESTATE, "adjoint: call fx_adjoint_forward before fx_adjoint_backward");
   FX_TRY(AdjTables());
@@ -343,16 +404,26 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
   const size_t n = (size_t)m_AdjN;
   float* V = m_AdjVec;
   const size_t tape_step = kTapeBytes * n + 8 * (size_t)m_AdjCells;
+  // The adjoint kernels run on their own stream; the system stream
+  // recomputes segment s-1 into the other tape buffer meanwhile.  Events:
+  // [b] buffer b holds its segment, [2 + b] the adjoint reading buffer b is
+  // done, [4] join.
+  FX_HIP(hipEventRecord(m_AdjEv[4], m_Stream), "adjoint");
+  FX_HIP(hipStreamWaitEvent(m_AdjStream, m_AdjEv[4], 0), "adjoint");
+  for (int b = 0; b < 2; b++) {
+    FX_HIP(hipEventRecord(m_AdjEv[2 + b], m_AdjStream), "adjoint");
+    FX_HIP(hipEventRecord(m_AdjEv[b], m_Stream), "adjoint");
+  }
   bool first = true;
+  if (m_AdjSegs > 0 && !m_AdjTapeValid) FX_TRY(AdjRecompute(m_AdjSegs - 1));
   for (int seg = m_AdjSegs - 1; seg >= 0; seg--) {
+    if (seg > 0 && m_AdjBufs == 2) FX_TRY(AdjRecompute(seg - 1));  // beside the adjoint of this segment
+    const int b = seg % m_AdjBufs;
+    FX_HIP(hipStreamWaitEvent(m_AdjStream, m_AdjEv[b], 0), "adjoint");
     const int s0 = seg * m_AdjEvery;
     const int s1 = (s0 + m_AdjEvery < m_AdjSteps) ? s0 + m_AdjEvery : m_AdjSteps;
-    if (!(seg == m_AdjSegs - 1 && m_AdjTapeValid)) {
-      FX_TRY(AdjRestoreCheckpoint(seg));
-      for (int t = s0; t < s1; t++) FX_TRY(AdjStep(t - s0));
-    }
     for (int t = s1 - 1; t >= s0; t--) {
-      const int k = t - s0;
+      const int k = b * m_AdjTapeSteps + t - s0;
       const char* T = m_AdjTape + tape_step * (size_t)k;
       AdjArgs A = {};
       A.params = m_AdjParams;
@@ -367,7 +438,7 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
       A.gcell = T + 56 * n;
       A.cnt = T + 60 * n;
       A.off = T + 60 * n + 4 * (size_t)m_AdjCells;
-      A.target = V + 19 * n;
+      A.target = V + 22 * n;
       A.gx = V;
       A.gv = V + 3 * n;
       A.ge = V + 6 * n;
@@ -382,14 +453,28 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
       A.nbits = (int)n;
       A.time = m_AdjTapeTime[k];
       A.accumulate = first ? 0 : 1;
-      FX_TRY(AdjLaunch(ADJ_ADVANCE, &A));
-      FX_TRY(AdjLaunch(ADJ_FORCE, &A));
-      FX_TRY(AdjLaunch(ADJ_DENSITY, &A));
+      FX_TRY(AdjLaunch(ADJ_ADVANCE, &A, m_AdjStream));
+      FX_TRY(AdjLaunch(ADJ_FORCE, &A, m_AdjStream));
+      FX_TRY(AdjLaunch(ADJ_DENSITY, &A, m_AdjStream));
       first = false;
     }
+    FX_HIP(hipEventRecord(m_AdjEv[2 + b], m_AdjStream), "adjoint");
+    if (seg > 0 && m_AdjBufs == 1) FX_TRY(AdjRecompute(seg - 1));  // one buffer: after its adjoint
   }
-  // the system continues from the end of the rollout
+  FX_HIP(hipEventRecord(m_AdjEv[4], m_AdjStream), "adjoint");
+  FX_HIP(hipStreamWaitEvent(m_Stream, m_AdjEv[4], 0), "adjoint");
+  // the system continues from the end of the rollout, with its original ids;
+  // the gradients go back to original-id order (into the gF..ges scratch)
   FX_TRY(AdjRestoreCheckpoint(m_AdjSegs));
+  {
+    void* fid = m_Points.gpu(FID);
+    float* out = V + 9 * n;
+    int nn = m_AdjN;
+    void* a[] = {&fid, &m_AdjId0, &nn};
+    FX_TRY(launch_n(m_AdjFunc[ADJ_FRAME_OUT], kAdjNames[ADJ_FRAME_OUT], m_Stream, nn, a));
+    void* b[] = {&V, &out, &m_AdjId0, &nn};
+    FX_TRY(launch_n(m_AdjFunc[ADJ_UNFRAME], kAdjNames[ADJ_UNFRAME], m_Stream, nn, b));
+  }
   m_AdjTapeValid = m_AdjSegs <= 1;
   m_AdjLossDone = false;
   std::vector<double> acc(3 * (size_t)m_AdjScenes, 0.0);
@@ -405,9 +490,9 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
     FX_HIP(hipMemcpyAsync(acc.data(), m_AdjAccum, acc.size() * sizeof(double), hipMemcpyDeviceToHost, m_Stream),
            "adjoint");
   const size_t vb = sizeof(float) * 3 * n;
-  if (gpos) FX_HIP(hipMemcpyAsync(gpos, V, vb, hipMemcpyDeviceToHost, m_Stream), "adjoint");
-  if (gvel) FX_HIP(hipMemcpyAsync(gvel, V + 3 * n, vb, hipMemcpyDeviceToHost, m_Stream), "adjoint");
-  if (gveval) FX_HIP(hipMemcpyAsync(gveval, V + 6 * n, vb, hipMemcpyDeviceToHost, m_Stream), "adjoint");
+  if (gpos) FX_HIP(hipMemcpyAsync(gpos, V + 9 * n, vb, hipMemcpyDeviceToHost, m_Stream), "adjoint");
+  if (gvel) FX_HIP(hipMemcpyAsync(gvel, V + 12 * n, vb, hipMemcpyDeviceToHost, m_Stream), "adjoint");
+  if (gveval) FX_HIP(hipMemcpyAsync(gveval, V + 15 * n, vb, hipMemcpyDeviceToHost, m_Stream), "adjoint");
   FX_HIP(hipStreamSynchronize(m_Stream), "adjoint");
   for (int s = 0; s < m_AdjScenes; s++) {
     if (gk) gk[s] = acc[3 * s + 1];

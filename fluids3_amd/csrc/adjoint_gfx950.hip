@@ -145,6 +145,35 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjLoss(AdjArgs A) {
   if (threadIdx.x == 0) A.partial[3 * blockIdx.x] = l;
 }
 
+// ------------------------------------------------------------ rollout frame
+// For the length of a rollout the particles are numbered by their slot at its
+// start (FID := slot), so the adjoint variables, indexed by FID, sit close to
+// each step's slot order (particles move little between steps) instead of in
+// the arbitrary order of the original ids.  adjFrameIn saves the original ids,
+// renumbers and moves the target into the frame; adjFrameOut restores the
+// ids of the current state; adjUnframe moves the three gradients back to
+// original-id order.
+extern "C" __global__ void adjFrameIn(uint* fid, uint* id0, const fx_float3* target, fx_float3* target_f, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint id = fid[s];
+  id0[s] = id;
+  fid[s] = (uint)s;
+  target_f[s] = target[id];
+}
+
+extern "C" __global__ void adjFrameOut(uint* fid, const uint* id0, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) fid[s] = id0[fid[s]];
+}
+
+extern "C" __global__ void adjUnframe(const fx_float3* g, fx_float3* out, const uint* id0, int n) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const size_t id = id0[a];
+  for (int v = 0; v < 3; v++) out[(size_t)v * n + id] = g[(size_t)v * n + a];
+}
+
 // ---------------------------------------------------------------- advance
 extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) {
   const Slot S = slot_of(A);
@@ -155,7 +184,6 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) 
   const fx_float3 gxo = A.gx[id], gvo = A.gv[id], geo = A.ge[id];
   if (A.gcell[s] == FX_GRID_UNDEF) {  // parked: pos and vel become constants, veval passes through
     const fx_float3 z = {0.0f, 0.0f, 0.0f};
-    A.gF[s] = z;
     A.gxs[s] = z;
     A.ges[s] = geo;
     A.gv[id] = z;
@@ -245,7 +273,6 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) 
     gaz = (gaz - az * d) * sa;
   }
   A.gv[id] = fx_float3{gvx, gvy, gvz};
-  A.gF[s] = fx_float3{gax * m, gay * m, gaz * m};
   A.arec[3 * (size_t)s] = make_float4(pos.x, pos.y, pos.z, A.rho[s]);
   A.arec[3 * (size_t)s + 1] = make_float4(ve.x, ve.y, ve.z, 0.0f);
   A.arec[3 * (size_t)s + 2] = make_float4(gax * m, gay * m, gaz * m, 0.0f);

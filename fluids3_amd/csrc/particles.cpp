@@ -36,6 +36,9 @@ static void free_batch(void*& a, void*& b, void*& c, std::map<int, std::pair<voi
 Particles::~Particles() {
   ReleaseGraph();
   AdjRelease();
+  for (hipEvent_t& e : m_AdjEv)
+    if (e) (void)hipEventDestroy(e);
+  if (m_AdjStream) (void)hipStreamDestroy(m_AdjStream);
   if (m_AdjModule) (void)hipModuleUnload(m_AdjModule);
   free_batch(m_DParams, m_DPBase, m_DCBase, m_BTab);
   m_Points.DeleteAllBuffers();

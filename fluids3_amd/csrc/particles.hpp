@@ -139,7 +139,8 @@ class Particles {
   int AdjStep(int record_slot);
   int AdjSaveCheckpoint(int k);
   int AdjRestoreCheckpoint(int k);
-  int AdjLaunch(int fid, void* args);
+  int AdjLaunch(int fid, void* args, hipStream_t st);
+  int AdjRecompute(int seg);
 
   hipModule_t m_Module = nullptr;
   hipFunction_t m_Func[FUNC_MAX] = {};
@@ -181,7 +182,10 @@ class Particles {
   // adjoint (reverse mode)
   std::string m_CodePath;
   hipModule_t m_AdjModule = nullptr;
-  hipFunction_t m_AdjFunc[5] = {};
+  hipStream_t m_AdjStream = nullptr;     // the backward's adjoint kernels (default priority)
+  hipEvent_t m_AdjEv[5] = {};            // per tape buffer: recomputed [0,1], adjoint done [2,3]; join [4]
+  int m_AdjBufs = 1, m_AdjTapeSteps = 0;  // tape buffers (2 when segments are recomputed) and slots in each
+  hipFunction_t m_AdjFunc[8] = {};
   int m_AdjSteps = 0, m_AdjEvery = 0, m_AdjSegs = 0, m_AdjN = 0, m_AdjCells = 0;
   bool m_AdjTapeValid = false, m_AdjLossDone = false;
   int64_t m_AdjBytes = 0;
@@ -194,7 +198,9 @@ class Particles {
   void* m_AdjArec = nullptr;             // 48 B x N pair records of the force adjoint
   void* m_AdjDrec = nullptr;             // 16 B x N pair records of the density adjoint
   void* m_AdjTabs = nullptr;             // FAccel tables routing ANBRBITS into the tape
-  float* m_AdjVec = nullptr;             // gx, gv, ge, gF, gxs, ges (3N floats each), grho (N), target (3N)
+  float* m_AdjVec = nullptr;             // gx, gv, ge, gF, gxs, ges (3N floats each), grho (N), target,
+                                         // target in the rollout frame (3N each)
+  uint32_t* m_AdjId0 = nullptr;          // original id of each slot at the rollout's start
   double* m_AdjPartial = nullptr;        // 3 per block
   double* m_AdjAccum = nullptr;          // 3 per scene
   void* m_AdjParams = nullptr;           // FParams_t per scene

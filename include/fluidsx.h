@@ -222,7 +222,8 @@ int fx_autotune(fx_system* sys, int32_t nsteps, int32_t* threads_out, double* ms
  * checkpoint_every <= 0 chooses the fewest segments, of equal length, whose
  * tape and checkpoints fit in 60 % of the device memory free at the call (the
  * backward recomputes all segments but the last; a rollout in one segment
- * needs no recomputation). */
+ * needs no recomputation).  When a second tape buffer also fits, the backward
+ * recomputes each segment while the adjoint of the next one runs. */
 int fx_adjoint_configure(fx_system* sys, const char* code_object_path, int32_t nsteps,
                          int32_t checkpoint_every);
 /* The segment length in use (the chosen one when configured with <= 0). */

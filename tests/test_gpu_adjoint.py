@@ -139,6 +139,23 @@ def test_adjoint_checkpoint_spacing_invariance(ps):
             assert g[k][0] == ref[k][0], (every, k)
 
 
+def test_adjoint_one_tape_buffer(ps, monkeypatch):
+    """With one tape buffer (kept when a second does not fit the memory
+    budget) the backward recomputes each segment after the adjoint of the
+    next one instead of beside it: same bits."""
+    sc = scenes.scaled_dam_break(5000)
+    T = 9
+    target = target_positions(ps, sc, T)
+    ref = gpu_grad(ps, sc, T, T, target)
+    monkeypatch.setenv("FX_ADJOINT_TAPE_BUFFERS", "1")
+    for every in (2, 4):
+        g = gpu_grad(ps, sc, T, every, target)
+        for k in ("gpos", "gvel", "gveval"):
+            assert bits_equal(g[k], ref[k]), (every, k)
+        for k in ("loss", "gvisc", "gk"):
+            assert g[k][0] == ref[k][0], (every, k)
+
+
 def test_adjoint_leaves_the_rollout_end_state(ps):
     sc = scenes.scaled_dam_break(4000)
     T = 7
