@@ -245,8 +245,8 @@ def main():
     # forward+adjoint rollout beside the plain step, same start state and window
     adjoint = None
     if args.workload == "dambreak_4m" and args.adjoint_steps > 0:
-        K = args.checkpoint_every if args.checkpoint_every is not None else 0
-        adjoint = adjoint_ratio(ps, restart, W, args.adjoint_steps, K, use_graph)
+        seg = args.checkpoint_every if args.checkpoint_every is not None else 0
+        adjoint = adjoint_ratio(ps, restart, W, args.adjoint_steps, seg, use_graph)
 
     cpu = None
     if want_cpu:

@@ -14,7 +14,7 @@ import os
 import numpy as np
 import pytest
 
-from fluids3_amd import Particles, abi, derive_params, fill_points, scene_spacing, scenes
+from fluids3_amd import Particles, abi, derive_params, fill_points, scenes
 from fluids3_amd.rollout import target_positions
 from oracle import oracle
 from parity import bits_equal
@@ -76,15 +76,17 @@ def test_adjoint_stencil_widths(ps, gd, srch):
 
 
 def test_adjoint_dense_block_without_bitmask(ps):
-    # ~180 particles per cell: interior particles need more than FX_NBR_WORDS
+    # ~300 particles per cell: interior particles need more than FX_NBR_WORDS
     # bitmask words, so their pairs are found by testing candidates in place
-    sc = scenes.dam_break(6000, (-40, 0, -40), (40, 60, 40), (-4, 4, -4), (4, 12, 4))
+    sc = scenes.dam_break(10000, (-40, 0, -40), (40, 60, 40), (-4, 4, -4), (4, 12, 4))
     T = 3
     target = target_positions(ps, sc, T)
     g = gpu_grad(ps, sc, T, 2, target)
     P = derive_params(sc)
     pos, clr = fill_points(sc)
-    assert scene_spacing(sc) < 0.5
+    srt = oracle.run(P, oracle.make_state(P, pos, clr=clr), 1,
+                     oracle.STAGE_INSERT | oracle.STAGE_SCAN | oracle.STAGE_SORT)
+    assert oracle.pairs_grid(P, srt)[0].max() > 32 * abi.NBR_WORDS
     r = oracle.grad(P, {"pos": pos, "clr": clr}, T, target, fp64=False, threads=THREADS)
     for k in ("gpos", "gvel", "gveval"):
         assert bits_equal(g[k], r[k]), (k, np.abs(g[k] - r[k]).max())

@@ -207,6 +207,32 @@ def test_stencil_widths(ps, gd, srch, threads):
     compare_state(g, ref)
 
 
+@pytest.mark.parametrize("threads", [64, 128, 256, 512])
+def test_dense_cells_without_bitmask(ps, threads):
+    """~300 particles per cell: interior particles have more candidates than
+    the bitmask holds (FX_NBR_WORDS words), so the force stage tests them in
+    place beside the word queue; row runs exceed one density tile, so the
+    density stage takes its chunked path.  Bit-exact with the oracle over 2
+    steps at every force path's block size."""
+    sc = scenes.dam_break(10000, (-40, 0, -40), (40, 60, 40), (-4, 4, -4), (4, 12, 4))
+    defaults = [ps.StageBlockSize(k) for k in range(abi.NUM_STAGES)]
+    ps.SetBlockSize(threads)
+    try:
+        ps.Restart(sc)
+        p = ps.params
+        s0 = ps.state()
+        ps.Run(2)
+        g = ps.state()
+    finally:
+        for k, t in enumerate(defaults):
+            ps.SetStageBlockSize(k, t)
+    ref = oracle.run(p, fresh_oracle(p, s0), 2, threads=THREADS)
+    compare_state(g, ref)
+    # the first step's sorted state: some particle has more candidates than the bitmask holds
+    srt = oracle.run(p, fresh_oracle(p, s0), 1, oracle.STAGE_INSERT | oracle.STAGE_SCAN | oracle.STAGE_SORT)
+    assert oracle.pairs_grid(p, srt)[0].max() > 32 * abi.NBR_WORDS
+
+
 @pytest.mark.parametrize("n", [1, 2, 63, 65, 1000, 4097])
 def test_ragged_counts(ps, n):
     sc = scenes.dambreak_16k(n=n)

@@ -41,6 +41,20 @@ def test_code_object_exports_the_module_contract():
         assert b"\x00" + k.encode() + b"\x00" in data, k
 
 
+def test_buffer_ids_match_the_header():
+    """Buffer ids and bitmask size in fluids3_amd.abi equal the #defines of
+    include/fluidsx_params.h."""
+    text = open(os.path.join(ROOT, "include", "fluidsx_params.h")).read()
+    defs = {k: int(v) for k, v in re.findall(r"#define (\w+) (\d+)\b", text)}
+    names = ["FPOS", "FVEL", "FCLR", "FVEVAL", "FFORCE", "FPRESS", "FGCELL", "FGNDX", "FID", "AGRID", "AGRIDCNT",
+             "AGRIDOFF", "AAUXARRAY1", "AAUXSCAN1", "AAUXARRAY2", "AAUXSCAN2", "ANBRBITS", "AFREC", "ATAILFLAG",
+             "ATAILSCAN", "ATAILAUX1", "ATAILAUXS1", "ATAILAUX2", "ATAILAUXS2"]
+    for k in names:
+        assert getattr(abi, k) == defs[k], k
+    assert abi.NBR_WORDS == defs["FX_NBR_WORDS"]
+    assert abi.GRID_UNDEF == defs["FX_GRID_UNDEF"]
+
+
 def test_params_layout_688_bytes():
     assert C.sizeof(abi.FParams) == 688
     assert abi.FParams.gridAdj.offset == 240 and abi.FParams.pmem.offset == 684
@@ -88,3 +102,14 @@ def test_create_without_code_object_fails_cleanly():
 
     with pytest.raises(FluidsError):
         Particles(code_object="/nonexistent/particles.hsaco")
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """Every box has one GPU: bench.py --gpus N > 1 exits with a message."""
+    import subprocess
+    import sys
+
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], cwd=ROOT,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0
+    assert "one GPU" in out.stdout + out.stderr
