@@ -126,6 +126,11 @@ int fx_sample_particles(fx_system* s, float* brick, const int32_t* res, const fl
   return s->p.SampleParticles(brick, res, bmin, bmax, scalar);
 }
 
+int fx_emit_particles(fx_system* s, float frame, int32_t emit) {
+  if (s == nullptr) return fx::fail(FX_EINVAL, "null system");
+  return s->p.EmitParticles(frame, emit);
+}
+
 int fx_save_points(fx_system* s, const char* path) {
   if (s == nullptr) return fx::fail(FX_EINVAL, "null system");
   return s->p.SavePoints(path);

@@ -18,7 +18,8 @@ static const char* kKernelNames[FUNC_MAX] = {
     "advanceParticles",  "prefixSum",        "prefixFixup",     "countingSortIndex",
     "fxFillUint",        "advanceParticlesClock", "fxClockTick",     "fxCountPairs",
     "insertParticlesBatch", "countingSortIndexBatch", "countingSortFullBatch", "computePressureBatch",
-    "computeForceBatch", "advanceParticlesBatch", "fxCountPairsBatch", "sampleParticles"};
+    "computeForceBatch", "advanceParticlesBatch", "fxCountPairsBatch", "sampleParticles",
+    "emitParticles"};
 
 Particles::Particles() { memset(&m_Params, 0, sizeof(m_Params)); }
 
@@ -688,6 +689,20 @@ int Particles::SampleParticles(float* brick, const int* res, const float* bmin, 
   (void)hipFree(d);
   if (e != hipSuccess) return fx::hip_fail(e, "sample_particles", "sampleParticles");
   return FX_OK;
+}
+
+int Particles::EmitParticles(float frame, int emit) {
+  // fluids4.0/fluid_system.cpp drives emitParticles once per frame with the
+  // frame number; one scene only
+  if (mNumPoints == 0) return fx::fail(FX_ESTATE, "emit_particles: call Restart first");
+  if (m_Batch) return fx::fail(FX_ESTATE, "emit_particles: one scene only");
+  if (emit < 0) return fx::fail(FX_EINVAL, "emit_particles: emit %d", emit);
+  if (emit > mNumPoints) emit = mNumPoints;
+  if (emit == 0) return FX_OK;
+  ReleaseGraph();
+  int n = mNumPoints;
+  void* args[] = {&frame, &emit, &n};
+  return Launch(FUNC_EMIT, (unsigned)((emit + 255) / 256), 256, args);
 }
 
 int Particles::CountPairs(int64_t* cand, int64_t* pairs) {

@@ -38,7 +38,8 @@
 #define FUNC_ADVANCE_B 17
 #define FUNC_COUNT_PAIRS_B 18
 #define FUNC_SAMPLE 19
-#define FUNC_MAX 20
+#define FUNC_EMIT 20
+#define FUNC_MAX 21
 
 #include <map>
 #include <string>
@@ -108,6 +109,7 @@ class Particles {
   int Autotune(int nsteps, int* chosen, double* stage_ms);  // autotune.cpp
   int SavePoints(const char* path);                         // points_io.cpp
   int SampleParticles(float* brick, const int* res, const float* bmin, const float* bmax, float scalar);
+  int EmitParticles(float frame, int emit);
   int LoadPoints(const char* path);
 
   // reverse mode (adjoint.cpp): checkpointed rollouts and their gradients

@@ -1096,6 +1096,46 @@ extern "C" __global__ void sampleParticles(float* brick, fx_int3 res, fx_float3 
   brick[out] = v * scalar;
 }
 
+// ------------------------------------------------------------------- emit
+
+// emitParticles (fluids4.0/fluid_kernels.cu:231-257): each of `emit` threads
+// respawns one particle at a point of the box (-170,10,-20)..(-190,60,20)
+// quantized to 1/256 per axis by three bytes of a 32-bit value v, with
+// velocity (5,-2,0) and zero veval and force.  The reference draws v from a
+// per-particle XORWOW state and respawns particle v & (numPnts-1), so two
+// threads can pick the same particle and the result depends on their order.
+// Here v is a counter-based SplitMix64 of (frame, thread index) and the
+// particles are a frame-seeded bijection of the thread index,
+// j = (A i + b) mod numPnts with A prime (2654435761): min(emit, numPnts)
+// distinct particles, whatever the thread order.
+__device__ __forceinline__ unsigned long long fx_splitmix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+extern "C" __global__ void emitParticles(float frame, int emit, int numPnts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= emit || i >= numPnts) return;
+  const unsigned long long h = fx_splitmix64((unsigned long long)__float_as_uint(frame));
+  const uint v = (uint)(fx_splitmix64(h ^ fx_splitmix64((unsigned long long)i)) >> 32);
+  const unsigned long long n = (unsigned long long)numPnts;
+  const uint j = (uint)((2654435761ull * (unsigned long long)i + h % n) % n);
+  const fx_float3 bmin = {-170.0f, 10.0f, -20.0f}, bmax = {-190.0f, 60.0f, 20.0f};
+  const float ux = (float)(v & 0xFFu) / 256.0f;
+  const float uy = (float)((v >> 8) & 0xFFu) / 256.0f;
+  const float uz = (float)((v >> 16) & 0xFFu) / 256.0f;
+  fx_float3 p;
+  p.x = bmin.x + ux * (bmax.x - bmin.x);
+  p.y = bmin.y + uy * (bmax.y - bmin.y);
+  p.z = bmin.z + uz * (bmax.z - bmin.z);
+  pbuf<fx_float3>(FPnts, FPOS)[j] = p;
+  pbuf<fx_float3>(FPnts, FVEVAL)[j] = fx_float3{0.0f, 0.0f, 0.0f};
+  pbuf<fx_float3>(FPnts, FVEL)[j] = fx_float3{5.0f, -2.0f, 0.0f};
+  pbuf<fx_float3>(FPnts, FFORCE)[j] = fx_float3{0.0f, 0.0f, 0.0f};
+}
+
 // ----------------------------------------------------------------- advance
 
 // Integrator (particles.cu:237-322): penalty walls on bound_min/bound_max

@@ -65,6 +65,7 @@ SIGNATURES = {
                                       C.POINTER(C.c_double)]),
     "fx_adjoint_memory": (C.c_int64, [_P]),
     "fx_adjoint_segment": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "fx_emit_particles": (C.c_int, [_P, C.c_float, C.c_int32]),
     "fx_last_error": (C.c_char_p, []),
     "fx_version": (C.c_char_p, []),
 }
@@ -262,6 +263,10 @@ class Particles:
         check(lib().fx_sample_particles(self._h, out.ctypes.data, r.ctypes.data, lo.ctypes.data, hi.ctypes.data,
                                         float(scalar)), "fx_sample_particles")
         return out
+
+    def EmitParticles(self, frame, emit):
+        """Respawns min(emit, n) particles in the emitter box (emitParticles); frame seeds the draw."""
+        check(lib().fx_emit_particles(self._h, float(frame), int(emit)), "fx_emit_particles")
 
     def SavePoints(self, path):
         """Writes the particle state as a .pts file (Fluids 4.0 layout plus veval and id)."""

@@ -188,6 +188,14 @@ int fx_count_pairs(fx_system* sys, int64_t* candidates, int64_t* pairs);
  * One scene only. */
 int fx_sample_particles(fx_system* sys, float* brick, const int32_t* res, const float* bmin, const float* bmax,
                         float scalar);
+/* Respawns min(emit, particles) particles at points of the box
+ * (-170,10,-20)..(-190,60,20) quantized to 1/256 per axis, velocity (5,-2,0),
+ * zero veval and force (emitParticles, fluids4.0/fluid_kernels.cu:231-257).
+ * The draws are a counter-based SplitMix64 of (frame, thread) and the
+ * respawned particles a frame-seeded bijection of the thread index (distinct
+ * particles, a result independent of thread order), where the reference uses
+ * per-particle XORWOW states and index v & (n-1).  One scene only. */
+int fx_emit_particles(fx_system* sys, float frame, int32_t emit);
 
 /* Launch geometry: threads per block of the per-particle kernels (64..1024,
  * multiple of 64), for every stage or for one.  Results do not depend on it. */

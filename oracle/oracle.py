@@ -58,6 +58,10 @@ def lib():
                            C.c_int]
         L.orc_sample.restype = C.c_int
         L.orc_sample.argtypes = [P, C.c_int, P, P, P, P, P, P, C.c_float, P]
+        L.orc_splitmix64.restype = C.c_uint64
+        L.orc_splitmix64.argtypes = [C.c_uint64]
+        L.orc_emit.restype = C.c_int
+        L.orc_emit.argtypes = [C.c_int, C.c_float, C.c_int, P, P, P, P]
         L.orc_grad_mixed.restype = C.c_int
         L.orc_grad_mixed.argtypes = [P, C.c_int, P, P, P, P, P, C.c_float, C.c_int, P, C.c_int, P, P, P, P, P, P]
         _lib = L
@@ -236,3 +240,20 @@ def sample(params, state, res, bmin, bmax, scalar=1.0):
     lib().orc_sample(C.byref(params), pos.shape[0], _ptr(pos), _ptr(cnt), _ptr(off), _ptr(r), _ptr(lo), _ptr(hi),
                      float(scalar), _ptr(out))
     return out
+
+
+def splitmix64(x):
+    """One SplitMix64 step from state x (the hash behind the emitter's draws)."""
+    return int(lib().orc_splitmix64(int(x) & 0xFFFFFFFFFFFFFFFF))
+
+
+def emit(state, frame, count):
+    """Oracle emitParticles on copies of pos/vel/veval/force: returns the four (n, 3) float32 arrays."""
+    pos = np.array(state["pos"], np.float32, order="C", copy=True)
+    n = pos.shape[0]
+    vel = np.array(state.get("vel", np.zeros((n, 3))), np.float32, order="C", copy=True)
+    veval = np.array(state.get("veval", np.zeros((n, 3))), np.float32, order="C", copy=True)
+    force = np.array(state.get("force", np.zeros((n, 3))), np.float32, order="C", copy=True)
+    lib().orc_emit(n, float(frame), int(count), _ptr(pos), _ptr(vel), _ptr(veval), _ptr(force))
+    return pos, vel, veval, force
+

@@ -18,6 +18,8 @@
 //                                                     both limiters, GRID_UNDEF park
 //   orc_pairs_*           neighbour counts          pinned: brute force
 //   orc_sample            volume splat              pinned: single-particle closed form
+//   orc_emit              emitter respawn           pinned: SplitMix64 known answer, distinct
+//                                                     targets, box lattice, velocity
 //   orc_grad_*            adjoint of a rollout      pinned (FP64): central differences of
 //                                                     orc_loss_f64 in every input kind,
 //                                                     a two-particle closed form
@@ -277,6 +279,43 @@ int orc_grad_mixed(const FParams_t* P, int n, const float* pos, const float* vel
   }
   *gvisc = G.gvisc;
   *gk = G.gk;
+  return 0;
+}
+
+// SplitMix64 (Steele, Lea, Flood, "Fast splittable pseudorandom number
+// generators", OOPSLA 2014): one step from state x.
+uint64_t orc_splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// emitParticles (fluids4.0/fluid_kernels.cu:231-257) with the draw of the
+// GPU kernel: thread i < min(emit, n) takes v = high word of
+// splitmix(h ^ splitmix(i)), h = splitmix(bits of frame), and respawns
+// particle (2654435761 i + h mod n) mod n at bmin + (byte/256)(bmax - bmin)
+// per axis with velocity (5,-2,0), zero veval and force.
+int orc_emit(int n, float frame, int emit, float* pos, float* vel, float* veval, float* force) {
+  uint32_t fb;
+  std::memcpy(&fb, &frame, 4);
+  const uint64_t h = orc_splitmix64((uint64_t)fb);
+  const int m = emit < n ? emit : n;
+  const float bmin[3] = {-170.0f, 10.0f, -20.0f}, bmax[3] = {-190.0f, 60.0f, 20.0f};
+  for (int i = 0; i < m; i++) {
+    const uint32_t v = (uint32_t)(orc_splitmix64(h ^ orc_splitmix64((uint64_t)i)) >> 32);
+    const uint64_t j = (2654435761ull * (uint64_t)i + h % (uint64_t)n) % (uint64_t)n;
+    for (int a = 0; a < 3; a++) {
+      const float u = (float)((v >> (8 * a)) & 0xFFu) / 256.0f;
+      const float t = u * (bmax[a] - bmin[a]);
+      pos[3 * j + a] = bmin[a] + t;
+      veval[3 * j + a] = 0.0f;
+      force[3 * j + a] = 0.0f;
+    }
+    vel[3 * j] = 5.0f;
+    vel[3 * j + 1] = -2.0f;
+    vel[3 * j + 2] = 0.0f;
+  }
   return 0;
 }
 

@@ -124,3 +124,24 @@ def test_sample_particles_single_particle_closed_form(ps):
     assert inside.sum() > 50
     assert np.allclose(g[inside], expect[inside], rtol=1e-5, atol=0)
     assert np.all(g[~inside] == 0)
+
+
+@pytest.mark.parametrize("count", [1, 300, 20000])
+def test_emit_particles_against_oracle(ps, count):
+    """emitParticles bit-exact with the oracle's respawn (the same draws),
+    then one more step runs on the respawned state."""
+    from oracle import oracle
+
+    sc = scenes.dambreak_16k()
+    ps.Restart(sc)
+    ps.Run(2)
+    st = ps.state()
+    ps.EmitParticles(7.0, count)
+    g = ps.state()
+    pos, vel, veval, force = oracle.emit(st, 7.0, count)
+    for k, r in (("pos", pos), ("vel", vel), ("veval", veval), ("force", force)):
+        assert bits_equal(g[k], r), k
+    moved = np.any(g["pos"] != st["pos"], axis=1)
+    assert moved.sum() == min(count, 16384)
+    ps.Run(1)
+    assert np.isfinite(ps.state()["pos"]).all()
