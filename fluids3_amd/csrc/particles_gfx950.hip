@@ -799,7 +799,7 @@ __device__ __forceinline__ void force_impl(const FParams_t& P, const Span S, int
 // write per word, and a deep queue of words keeps nearly every lane busy in
 // the batches, so nearly every pair gather carries a pair.
 #ifndef FW_QBYTES
-#define FW_QBYTES 8192  // LDS queue per block: (word, base) pairs
+#define FW_QBYTES 16384  // LDS queue per block: (word, base) pairs, 4 words per lane at 512 threads
 #endif
 
 template <int R, int S>
@@ -957,14 +957,14 @@ __device__ __forceinline__ void force_q_impl(const FParams_t& P, const Span S, i
     }
   }
   const bool use = act && !ovf;
-  const int K = FW_QBYTES / (8 * nt);
+  const int K = min(FW_QBYTES / (8 * nt), 16);
   force_walk_q<R, 2>(P, I, bits, (size_t)N, i, use ? nwords : 0, E_, G, lq, lq + K * nt, K, fr, fr + 1, use);
   if (act) pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{I.fx, I.fy, I.fz};
 }
 
 // Dispatch: the word queue needs the density stage's records and at least
-// 4 LDS words per lane (blocks of up to 256 threads); otherwise the ring or
-// candidate-testing pass.
+// 4 LDS words per lane (blocks of up to 512 threads, the reference's launch
+// geometry); otherwise the ring or candidate-testing pass.
 __device__ __forceinline__ void force_q(const FParams_t& P, const Span S, int N, uint* lq) {
   if (FAccel.mbuf[ANBRBITS] == 0 || FAccel.mbuf[AFREC] == 0 || FW_QBYTES / (8 * (int)blockDim.x) < 4 ||
       P.gridSrch > 3) {

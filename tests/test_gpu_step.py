@@ -165,8 +165,8 @@ def test_graph_equals_eager(ps):
 @pytest.mark.parametrize("threads", [64, 128, 512, 1024])
 def test_block_size_invariance(ps, threads):
     """Results do not depend on the launch geometry (the force stage's word
-    queue holds 16, 8 or 4 LDS words per lane at 64, 128 and 256 threads per
-    block; larger blocks evaluate pairs in place)."""
+    queue holds 16, 16, 8 or 4 LDS words per lane at 64, 128, 256 and 512
+    threads per block; 1024-thread blocks evaluate pairs in place)."""
     sc = scenes.dambreak_16k()
     defaults = [ps.StageBlockSize(k) for k in range(abi.NUM_STAGES)]
     ps.Restart(sc)
