@@ -1,8 +1,8 @@
-"""bench.py's output contract: one JSON line whose fields follow the
-benchmark driver's contract (steps and warm-up as asked, the metric, a
-roofline object for the dominant kernel and the CPU baseline beside it),
-checked on the 16,384-particle dam break with a handful of steps.  bench.py
-runs as a child process, as the driver runs it."""
+"""bench.py's output: one JSON line with the benchmark fields (steps and
+warm-up as asked, the metric, a roofline object for the dominant kernel and
+the CPU baseline beside it), checked on the 16,384-particle dam break with a
+handful of steps.  bench.py runs as a child process, as it is run from the
+command line."""
 import json
 import os
 import subprocess
