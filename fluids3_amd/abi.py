@@ -13,7 +13,7 @@ FPOS, FVEL, FCLR, FVEVAL, FFORCE, FPRESS, FGCELL, FGNDX, FID = range(9)
 # grid buffers (fluid.h:25-31)
 AGRID, AGRIDCNT, AGRIDOFF, AAUXARRAY1, AAUXSCAN1, AAUXARRAY2, AAUXSCAN2 = range(7)
 ANBRBITS, AFREC, ATAILFLAG, ATAILSCAN, ATAILAUX1, ATAILAUXS1, ATAILAUX2, ATAILAUXS2 = range(7, 15)
-NBR_WORDS = 128  # FX_NBR_WORDS: bitmask words per particle (4096 candidates)
+NBR_WORDS = 256  # FX_NBR_WORDS: bitmask words per particle (8192 candidates)
 
 # stages of Particles::Run (particles.cpp:382-394)
 STAGE_INSERT, STAGE_PREFIX, STAGE_SORT, STAGE_PRESSURE, STAGE_FORCE, STAGE_ADVANCE = range(6)

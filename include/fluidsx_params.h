@@ -69,7 +69,7 @@ extern "C" {
 #define ATAILAUX2 13
 #define ATAILAUXS2 14
 #define FX_NUM_ABUF 15
-#define FX_NBR_WORDS 128 /* bitmask words per particle (4096 candidates) */
+#define FX_NBR_WORDS 256 /* bitmask words per particle (8192 candidates) */
 
 #define FX_DMAXBUF 16 /* datax.h:37 */
 

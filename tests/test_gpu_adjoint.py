@@ -76,9 +76,9 @@ def test_adjoint_stencil_widths(ps, gd, srch):
 
 
 def test_adjoint_dense_block_without_bitmask(ps):
-    # ~300 particles per cell: interior particles need more than FX_NBR_WORDS
+    # ~500 particles per cell: interior particles need more than FX_NBR_WORDS
     # bitmask words, so their pairs are found by testing candidates in place
-    sc = scenes.dam_break(10000, (-40, 0, -40), (40, 60, 40), (-4, 4, -4), (4, 12, 4))
+    sc = scenes.dam_break(16000, (-40, 0, -40), (40, 60, 40), (-4, 4, -4), (4, 12, 4))
     T = 3
     target = target_positions(ps, sc, T)
     g = gpu_grad(ps, sc, T, 2, target)

@@ -209,12 +209,12 @@ def test_stencil_widths(ps, gd, srch, threads):
 
 @pytest.mark.parametrize("threads", [64, 128, 256, 512])
 def test_dense_cells_without_bitmask(ps, threads):
-    """~300 particles per cell: interior particles have more candidates than
+    """~500 particles per cell: interior particles have more candidates than
     the bitmask holds (FX_NBR_WORDS words), so the force stage tests them in
     place beside the word queue; row runs exceed one density tile, so the
     density stage takes its chunked path.  Bit-exact with the oracle over 2
     steps at every force path's block size."""
-    sc = scenes.dam_break(10000, (-40, 0, -40), (40, 60, 40), (-4, 4, -4), (4, 12, 4))
+    sc = scenes.dam_break(16000, (-40, 0, -40), (40, 60, 40), (-4, 4, -4), (4, 12, 4))
     defaults = [ps.StageBlockSize(k) for k in range(abi.NUM_STAGES)]
     ps.SetBlockSize(threads)
     try:
