@@ -386,7 +386,9 @@ __device__ __forceinline__ int stencil_base(const FParams_t& P, int gc) {
 // reads the part of the chunk that falls in its own row window.  Chunks are
 // visited in increasing order, so every lane still sees its candidates in the
 // reference's order.
+#ifndef TILE_P
 #define TILE_P 1024  // float4 (pos, 0) per candidate, 16 KB
+#endif
 
 struct BlockSpan {
   int h1;      // end of the block's in-range particles
