@@ -119,6 +119,32 @@ def test_stages_dambreak_16k(ps):
             assert bits_equal(a, b), k
 
 
+def test_caller_runs_one_stage_on_the_cpu(ps):
+    """The reference's per-function CPU/GPU choice (m_bGPU, particles.cpp:35)
+    composed by the caller: GPU insert .. force through the per-stage calls,
+    the advance on the host (here the oracle's) over the retrieved state,
+    Commit of pos/vel/veval and the clock moved by dt through UpdateParams.
+    Three such steps equal three all-GPU steps bit for bit."""
+    sc = scenes.dambreak_16k()
+    ps.Restart(sc)
+    ps.Run(3)
+    want = ps.state()
+    ps.Restart(sc)
+    for _ in range(3):
+        for stage in (ps.InsertParticles, ps.PrefixScanParticles, ps.CountingSort, ps.ComputePressure,
+                      ps.ComputeForce):
+            stage()
+        p = ps.params
+        r = oracle.run(p, snapshot_to_oracle(ps.state()), 1, oracle.STAGE_ADVANCE)
+        for buf, k in ((abi.FPOS, "pos"), (abi.FVEL, "vel"), (abi.FVEVAL, "veval")):
+            ps.Commit(buf, r[k])
+        p.time = np.float32(np.float32(p.time) + np.float32(p.dt))
+        ps.UpdateParams(p)
+    got = ps.state()
+    assert got["time"] == want["time"]
+    compare_state(got, want)
+
+
 def test_multi_step_bitexact_dambreak_16k(ps):
     sc = scenes.dambreak_16k()
     ps.Restart(sc)
