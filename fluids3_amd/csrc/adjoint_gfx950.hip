@@ -90,11 +90,31 @@ struct AdjArgs {
 struct Slot {
   const FParams_t* P;
   int i, p0, p1, scene;
+  int blk;  // logical block: entry of the block table and of the partials
 };
 
+// XCD-aware block order, as in the forward pair kernels: MI355X hands
+// consecutive workgroups to its eight XCDs round-robin, so runs of
+// ADJ_XCD_RUN consecutive logical blocks (neighbouring particles, which share
+// neighbours) are given to one XCD and the runs rotate over the XCDs.
+// Dispatch slot b is the k-th block of XCD x = b % 8 and runs logical block
+// (run * 8 + x) * ADJ_XCD_RUN + k % ADJ_XCD_RUN with run = k / ADJ_XCD_RUN;
+// blocks past the last full round keep their index.
+#ifndef ADJ_XCD_RUN
+#define ADJ_XCD_RUN 128
+#endif
+__device__ __forceinline__ int adj_block() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int round = 8 * ADJ_XCD_RUN;
+  if (b >= nb / round * round) return b;
+  const int x = b % 8, k = b / 8;
+  return ((k / ADJ_XCD_RUN) * 8 + x) * ADJ_XCD_RUN + k % ADJ_XCD_RUN;
+}
+
 __device__ __forceinline__ Slot slot_of(const AdjArgs& A) {
-  const int2 e = A.btab[blockIdx.x];
   Slot s;
+  s.blk = adj_block();
+  const int2 e = A.btab[s.blk];
   s.P = cparams(A.params + e.x);
   s.scene = e.x;
   s.p0 = A.pbase[e.x];
@@ -142,7 +162,7 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjLoss(AdjArgs A) {
     A.ge[id] = fx_float3{0.0f, 0.0f, 0.0f};
   }
   l = block_sum<ADJ_THREADS>(l, lds);
-  if (threadIdx.x == 0) A.partial[3 * blockIdx.x] = l;
+  if (threadIdx.x == 0) A.partial[3 * S.blk] = l;
 }
 
 // ------------------------------------------------------------ rollout frame
@@ -651,7 +671,7 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) AW_FORCE_ATTR adjForce
   const double sk = wave_sum((double)F.ak);
   const double sv = wave_sum((double)F.av);
   if (threadIdx.x == 0) {
-    double* o = A.partial + 3 * blockIdx.x;
+    double* o = A.partial + 3 * S.blk;
     o[1] = A.accumulate ? o[1] + sk : sk;
     o[2] = A.accumulate ? o[2] + sv : sv;
   }

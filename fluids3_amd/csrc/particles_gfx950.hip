@@ -126,7 +126,9 @@ __device__ __forceinline__ uint lanes_below(unsigned long long mask) {
 // of XCD x = b % 8; it runs logical block (run * 8 + x) * FX_XCD_RUN + k % RUN
 // with run = k / RUN.  Blocks past the last full round keep their index.
 #define FX_NUM_XCD 8
+#ifndef FX_XCD_RUN
 #define FX_XCD_RUN 32
+#endif
 __device__ __forceinline__ int xcd_block() {
   const int nb = (int)gridDim.x, b = (int)blockIdx.x;
   const int round = FX_NUM_XCD * FX_XCD_RUN;
@@ -554,13 +556,13 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
 
 extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
   __shared__ float4 tile[TILE_P];
-  pressure_impl(*cparams(&FParams), Span{0, pnum, 0, (int)blockIdx.x}, pnum, tile);
+  pressure_impl(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, tile);
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computePressureBatch(BatchArgs B, int ntotal) {
   __shared__ float4 tile[TILE_P];
   const FParams_t* P;
-  const Span S = batch_span(B, blockIdx.x, P);
+  const Span S = batch_span(B, xcd_block(), P);
   pressure_impl(*P, S, ntotal, tile);
 }
 
