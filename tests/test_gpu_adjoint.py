@@ -41,11 +41,14 @@ def gpu_grad(ps, scene, nsteps, every, target):
     return g
 
 
-def test_adjoint_bitexact_against_fp32_oracle(ps):
-    sc = scenes.scaled_dam_break(3000)
-    T = 12
+@pytest.mark.parametrize("n,T,every", [(3000, 12, 5), (80000, 4, 3)])
+def test_adjoint_bitexact_against_fp32_oracle(ps, n, T, every):
+    """Segments of 5, 5, 2 steps at 3,000 particles; at 80,000 particles the
+    one-wave blocks (1,250) reach past a full round of the XCD-aware block
+    order (8 x 128 blocks), so both the remapped blocks and the tail run."""
+    sc = scenes.scaled_dam_break(n)
     target = target_positions(ps, sc, T)
-    g = gpu_grad(ps, sc, T, 5, target)  # segments of 5, 5, 2 steps
+    g = gpu_grad(ps, sc, T, every, target)
     P = derive_params(sc)
     pos, clr = fill_points(sc)
     r = oracle.grad(P, {"pos": pos, "clr": clr}, T, target, fp64=False, threads=THREADS)
@@ -208,9 +211,12 @@ def test_adjoint_against_fp64_cpu_adjoints_16k(ps):
     print(f"loss {g['loss'][0]:.10e}, FP64 rollout {r['loss']:.10e}")
 
 
-def test_batch_adjoint_equals_single_scene_adjoints(ps):
-    sc = scenes.batch_scenes(count=4, lo=1500, hi=6000, seed=21)
-    T = 6
+@pytest.mark.parametrize("lo,hi,T", [(1500, 6000, 6), (15000, 40000, 3)])
+def test_batch_adjoint_equals_single_scene_adjoints(ps, lo, hi, T):
+    """The larger batch launches more than a full round of the XCD-aware block
+    order (8 x 128 one-wave blocks), so logical blocks of several scenes share
+    an XCD run and the per-scene FP64 partials are gathered by logical block."""
+    sc = scenes.batch_scenes(count=4, lo=lo, hi=hi, seed=21)
     targets = [target_positions(ps, s, T) for s in sc]
     ps.RestartBatch(sc)
     ps.AdjointConfigure(T, 4)
