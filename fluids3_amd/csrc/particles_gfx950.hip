@@ -791,7 +791,9 @@ __device__ __forceinline__ void force_impl(const FParams_t& P, const Span S, int
   if (act) pbuf<fx_float3>(FPnts, FFORCE)[i] = fx_float3{I.fx, I.fy, I.fz};
 }
 
+#ifndef FW_AHEAD
 #define FW_AHEAD 4  // bitmask words loaded ahead of their use
+#endif
 
 // Word-queue force.  Each lane reads its bitmask words as one stream (row
 // r's words follow row r-1's) and queues every non-zero word whole, with the
@@ -981,12 +983,16 @@ __device__ __forceinline__ void force_q(const FParams_t& P, const Span S, int N,
     force_q_impl<2>(P, S, N, lq);
 }
 
-extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
+#ifndef FW_ATTR
+#define FW_ATTR
+#endif
+
+extern "C" __global__ __launch_bounds__(1024) FW_ATTR void computeForce(int pnum) {
   __shared__ uint lq[FW_QBYTES / 4];
   force_q(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, lq);
 }
 
-extern "C" __global__ __launch_bounds__(1024) void computeForceBatch(BatchArgs B, int ntotal) {
+extern "C" __global__ __launch_bounds__(1024) FW_ATTR void computeForceBatch(BatchArgs B, int ntotal) {
   __shared__ uint lq[FW_QBYTES / 4];
   const FParams_t* P;
   const Span S = batch_span(B, xcd_block(), P);
