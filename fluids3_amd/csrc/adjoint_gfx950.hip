@@ -440,7 +440,7 @@ __device__ __forceinline__ void ring_walk(const FParams_t& P, const AdjArgs& A, 
 // every non-zero word whole, with the record index of its bit 0 -- two words
 // in registers, up to AW_QWORDS more in LDS; the wave then takes the same
 // number of pairs from every lane (as many as the shortest queue holds, more
-// only when a queue must make room), two per step, each lane's in candidate
+// only when a queue must make room), AW_STEP per step, each lane's in candidate
 // order.  Same contract as ring_walk; for stencil rows of SR cells, K LDS
 // words per lane (8 K bytes per block).
 #ifndef AW_QF
@@ -448,6 +448,9 @@ __device__ __forceinline__ void ring_walk(const FParams_t& P, const AdjArgs& A, 
 #endif
 #ifndef AW_QD
 #define AW_QD 8   // and in the density adjoint
+#endif
+#ifndef AW_STEP
+#define AW_STEP 2  // pairs taken per drain step
 #endif
 #ifndef AW_FORCE_ATTR
 #define AW_FORCE_ATTR
@@ -494,14 +497,15 @@ __device__ __forceinline__ void queue_walk(const FParams_t& P, const AdjArgs& A,
   };
   auto drain = [&](int f) {
     int e = 0;
-    for (; e + 2 <= f; e += 2) {
-      uint j0, j1;
-      const bool ok0 = take(j0);
-      const bool ok1 = take(j1);
-      op.eval(j0, ok0);
-      op.eval(j1, ok1);
+    for (; e + AW_STEP <= f; e += AW_STEP) {
+      uint j[AW_STEP];
+      bool ok[AW_STEP];
+#pragma unroll
+      for (int u = 0; u < AW_STEP; u++) ok[u] = take(j[u]);
+#pragma unroll
+      for (int u = 0; u < AW_STEP; u++) op.eval(j[u], ok[u]);
     }
-    if (e < f) {
+    for (; e < f; e++) {
       uint j;
       const bool ok = take(j);
       op.eval(j, ok);

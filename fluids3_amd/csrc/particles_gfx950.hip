@@ -800,10 +800,13 @@ __device__ __forceinline__ void force_impl(const FParams_t& P, const Span S, int
 // record index of its bit 0: two words in registers, up to K more in LDS.
 // The wave then evaluates pairs in batches: a batch takes the same number of
 // pairs from every lane (as many as the shortest queue holds, more only when
-// a queue must make room), each lane's pairs in candidate order, two pairs'
-// terms evaluated at a time.  Queuing words instead of pairs costs one LDS
+// a queue must make room), each lane's pairs in candidate order, FW_STEP
+// pairs' terms evaluated at a time.  Queuing words instead of pairs costs one LDS
 // write per word, and a deep queue of words keeps nearly every lane busy in
 // the batches, so nearly every pair gather carries a pair.
+#ifndef FW_STEP
+#define FW_STEP 3  // pair terms evaluated per drain step
+#endif
 #ifndef FW_QBYTES
 #define FW_QBYTES 16384  // LDS queue per block: (word, base) pairs, 4 words per lane at 512 threads
 #endif
@@ -836,19 +839,21 @@ __device__ __forceinline__ void force_walk_q(const FParams_t& P, ForceI& I, cons
     return ok;
   };
   // f pairs from every lane (a lane that runs out evaluates record 0 and
-  // discards it)
+  // discards it), FW_STEP pair terms evaluated at a time and added in order
   auto drain = [&](int f) {
     int e = 0;
-    for (; e + 2 <= f; e += 2) {
-      uint j0, j1;
-      const bool ok0 = take(j0);
-      const bool ok1 = take(j1);
-      const PairTerm q0 = pair_term(P, I, srcA[S * (size_t)j0], srcV[S * (size_t)j0]);
-      const PairTerm q1 = pair_term(P, I, srcA[S * (size_t)j1], srcV[S * (size_t)j1]);
-      pair_add_if(I, q0, ok0);
-      pair_add_if(I, q1, ok1);
+    for (; e + FW_STEP <= f; e += FW_STEP) {
+      uint j[FW_STEP];
+      bool ok[FW_STEP];
+      PairTerm q[FW_STEP];
+#pragma unroll
+      for (int u = 0; u < FW_STEP; u++) ok[u] = take(j[u]);
+#pragma unroll
+      for (int u = 0; u < FW_STEP; u++) q[u] = pair_term(P, I, srcA[S * (size_t)j[u]], srcV[S * (size_t)j[u]]);
+#pragma unroll
+      for (int u = 0; u < FW_STEP; u++) pair_add_if(I, q[u], ok[u]);
     }
-    if (e < f) {
+    for (; e < f; e++) {
       uint j;
       const bool ok = take(j);
       pair_add_if(I, pair_term(P, I, srcA[S * (size_t)j], srcV[S * (size_t)j]), ok);
