@@ -573,11 +573,28 @@ int Particles::LaunchGraph(int nlaunch) {
 
 // ---------------------------------------------------------------- transfers
 
+int Particles::CheckIds(const uint32_t* id, int64_t n, const char* who) {
+  std::vector<uint8_t> seen((size_t)n, 0);
+  for (int64_t k = 0; k < n; k++) {
+    if (id[k] >= (uint64_t)n || seen[id[k]])
+      return fx::fail(FX_EINVAL, "%s: ids are not a permutation of 0..%lld (entry %lld = %u)", who, (long long)n - 1,
+                      (long long)k, id[k]);
+    seen[id[k]] = 1;
+  }
+  return FX_OK;
+}
+
 int Particles::Upload(int buf, const void* host, int64_t count) {
   DataPtr* b = m_Points.GetBuffer(buf);
   if (b == nullptr || host == nullptr) return fx::fail(FX_EINVAL, "upload: unknown buffer %d", buf);
   if (count < 0 || (uint64_t)count > b->mMax) return fx::fail(FX_ERANGE, "upload: %lld elements exceed %llu",
                                                              (long long)count, (unsigned long long)b->mMax);
+  if (buf == FID) {
+    // ids index the adjoint's target and gradients: only a whole permutation of 0..n-1 is taken
+    if (count != mNumPoints)
+      return fx::fail(FX_EINVAL, "upload: FID takes all %d ids at once, not %lld", mNumPoints, (long long)count);
+    FX_TRY(CheckIds(static_cast<const uint32_t*>(host), count, "upload"));
+  }
   FX_HIP(hipStreamSynchronize(m_Stream), "upload");
   FX_HIP(hipMemcpyHtoD(b->mGpu, const_cast<void*>(host), (size_t)count * b->mStride), "upload");
   return FX_OK;

@@ -95,8 +95,9 @@ class Particles {
 
   // transfers (particles.cpp:703-711)
   int CommitAll();
-  int Upload(int buf, const void* host, int64_t count);
+  int Upload(int buf, const void* host, int64_t count);  // FID: a whole permutation of 0..n-1 only
   int Download(int buf, void* host, int64_t count);
+  static int CheckIds(const uint32_t* id, int64_t n, const char* who);
   int DownloadAccel(int buf, void* host, int64_t count);
   int Synchronize();
   int Scan(const uint32_t* in, uint32_t* out, int n);

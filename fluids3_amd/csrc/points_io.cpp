@@ -91,6 +91,7 @@ int Particles::LoadPoints(const char* path) {
       memcpy(d[4].data() + 4 * (size_t)i, &v, 4);
     }
   }
+  FX_TRY(CheckIds(reinterpret_cast<const uint32_t*>(d[4].data()), n, "load_points"));  // before any upload
   ReleaseGraph();
   FX_TRY(Upload(FPOS, d[0].data(), n));
   FX_TRY(Upload(FVEL, d[1].data(), n));

@@ -124,7 +124,9 @@ int32_t fx_cell_table_size(const fx_system* sys);
 /* Element size of particle buffer id (FPOS..FID), 0 for an unknown id. */
 int32_t fx_buffer_stride(int32_t buf);
 /* Copy count elements of particle buffer buf between host and device.
- * Synchronous with respect to the system's stream. */
+ * Synchronous with respect to the system's stream.  An upload of FID must
+ * hold all n ids and be a permutation of 0..n-1 (FX_EINVAL otherwise): the
+ * ids index the adjoint's target and gradients. */
 int fx_upload(fx_system* sys, int32_t buf, const void* host, int64_t count);
 int fx_download(fx_system* sys, int32_t buf, void* host, int64_t count);
 /* Grid buffers AGRID (N uints), AGRIDCNT / AGRIDOFF (gridTotal+1 uints). */

@@ -85,6 +85,34 @@ def test_load_fluids40_file_and_count_mismatch(ps, tmp_path):
         ps.LoadPoints(path)
 
 
+def test_ids_must_be_a_permutation(ps, tmp_path):
+    """FID indexes the adjoint's target and gradients: an upload or a .pts
+    file whose ids are not a permutation of 0..n-1 is refused, and a refused
+    file leaves the state as it was."""
+    sc = scenes.dambreak_16k()
+    ps.Restart(sc)
+    n = sc.num_points
+    ids = np.arange(n, dtype=np.uint32)
+    for bad in (np.where(ids == 7, 8, ids), np.where(ids == 7, n, ids), ids[:100]):
+        with pytest.raises(FluidsError):
+            ps.Commit(abi.FID, bad.astype(np.uint32))
+    ps.Commit(abi.FID, ids[::-1].copy())
+    assert np.array_equal(ps.Retrieve(abi.FID), ids[::-1])
+    ps.Restart(sc)
+    before = ps.state()
+    path = str(tmp_path / "dup.pts")
+    ps.SavePoints(path)
+    raw = bytearray(open(path, "rb").read())
+    tail = len(raw) - 4 * n  # the id field is the file's last n int32
+    raw[tail:tail + 4] = np.uint32(1).tobytes()  # id 0 becomes a second 1
+    open(path, "wb").write(bytes(raw))
+    with pytest.raises(FluidsError):
+        ps.LoadPoints(path)
+    after = ps.state()
+    for k in ("pos", "vel", "veval", "clr", "id"):
+        assert bits_equal(after[k], before[k]), k
+
+
 def test_sample_particles_against_oracle(ps):
     from oracle import oracle
     from parity import rel_err
