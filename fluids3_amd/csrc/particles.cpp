@@ -230,6 +230,7 @@ int Particles::FinishRestart(int n) {
   FX_TRY(UpdateParams());
   FX_TRY(CommitAll());
   m_Frame = 0;
+  m_Done = -1;
   return FX_OK;
 }
 
@@ -370,10 +371,13 @@ int Particles::InsertParticles() {
     unsigned nb;
     FxBatchArgs B = BatchFor(FX_STAGE_INSERT, &nb);
     void* bargs[] = {&B};
-    return Launch(FUNC_INSERT_B, nb, (unsigned)m_StageThreads[FX_STAGE_INSERT], bargs);
+    FX_TRY(Launch(FUNC_INSERT_B, nb, (unsigned)m_StageThreads[FX_STAGE_INSERT], bargs));
+  } else {
+    void* args[] = {&mNumPoints};
+    FX_TRY(LaunchP(FUNC_INSERT, FX_STAGE_INSERT, args));
   }
-  void* args[] = {&mNumPoints};
-  return LaunchP(FUNC_INSERT, FX_STAGE_INSERT, args);
+  m_Done = FX_STAGE_INSERT;
+  return FX_OK;
 }
 
 int Particles::PrefixScanParticles() {
@@ -382,8 +386,10 @@ int Particles::PrefixScanParticles() {
   FX_TRY(ScanLevels(m_Accel.gpu(AGRIDCNT), m_Accel.gpu(AGRIDOFF), m_Accel.gpu(AAUXARRAY1),
                     m_Accel.gpu(AAUXSCAN1), m_Accel.gpu(AAUXARRAY2), m_Accel.gpu(AAUXSCAN2),
                     m_ScanCells));
-  return ScanLevels(m_Accel.gpu(ATAILFLAG), m_Accel.gpu(ATAILSCAN), m_Accel.gpu(ATAILAUX1),
-                    m_Accel.gpu(ATAILAUXS1), m_Accel.gpu(ATAILAUX2), m_Accel.gpu(ATAILAUXS2), mNumPoints);
+  FX_TRY(ScanLevels(m_Accel.gpu(ATAILFLAG), m_Accel.gpu(ATAILSCAN), m_Accel.gpu(ATAILAUX1),
+                    m_Accel.gpu(ATAILAUXS1), m_Accel.gpu(ATAILAUX2), m_Accel.gpu(ATAILAUXS2), mNumPoints));
+  m_Done = FX_STAGE_PREFIX;
+  return FX_OK;
 }
 
 int Particles::ScanLevels(void* array1, void* scan1, void* array2, void* scan2, void* array3, void* scan3,
@@ -430,11 +436,14 @@ int Particles::CountingSort() {
     FxBatchArgs B = BatchFor(FX_STAGE_SORT, &nb);
     void* bargs[] = {&B};
     FX_TRY(Launch(FUNC_SORT_INDEX_B, nb, (unsigned)m_StageThreads[FX_STAGE_SORT], bargs));
-    return Launch(FUNC_SORT_FULL_B, nb, (unsigned)m_StageThreads[FX_STAGE_SORT], bargs);
+    FX_TRY(Launch(FUNC_SORT_FULL_B, nb, (unsigned)m_StageThreads[FX_STAGE_SORT], bargs));
+  } else {
+    void* args[] = {&mNumPoints};
+    FX_TRY(LaunchP(FUNC_SORT_INDEX, FX_STAGE_SORT, args));
+    FX_TRY(LaunchP(FUNC_COUNTING_SORT, FX_STAGE_SORT, args));
   }
-  void* args[] = {&mNumPoints};
-  FX_TRY(LaunchP(FUNC_SORT_INDEX, FX_STAGE_SORT, args));
-  return LaunchP(FUNC_COUNTING_SORT, FX_STAGE_SORT, args);
+  m_Done = FX_STAGE_SORT;
+  return FX_OK;
 }
 
 int Particles::ComputePressure() {
@@ -442,10 +451,13 @@ int Particles::ComputePressure() {
     unsigned nb;
     FxBatchArgs B = BatchFor(FX_STAGE_PRESSURE, &nb);
     void* bargs[] = {&B, &mNumPoints};
-    return Launch(FUNC_PRESS_B, nb, (unsigned)m_StageThreads[FX_STAGE_PRESSURE], bargs);
+    FX_TRY(Launch(FUNC_PRESS_B, nb, (unsigned)m_StageThreads[FX_STAGE_PRESSURE], bargs));
+  } else {
+    void* args[] = {&mNumPoints};
+    FX_TRY(LaunchP(FUNC_COMPUTE_PRESS, FX_STAGE_PRESSURE, args));
   }
-  void* args[] = {&mNumPoints};
-  return LaunchP(FUNC_COMPUTE_PRESS, FX_STAGE_PRESSURE, args);
+  m_Done = FX_STAGE_PRESSURE;
+  return FX_OK;
 }
 
 int Particles::ComputeForce() {
@@ -453,10 +465,13 @@ int Particles::ComputeForce() {
     unsigned nb;
     FxBatchArgs B = BatchFor(FX_STAGE_FORCE, &nb);
     void* bargs[] = {&B, &mNumPoints};
-    return Launch(FUNC_FORCE_B, nb, (unsigned)m_StageThreads[FX_STAGE_FORCE], bargs);
+    FX_TRY(Launch(FUNC_FORCE_B, nb, (unsigned)m_StageThreads[FX_STAGE_FORCE], bargs));
+  } else {
+    void* args[] = {&mNumPoints};
+    FX_TRY(LaunchP(FUNC_COMPUTE_FORCE, FX_STAGE_FORCE, args));
   }
-  void* args[] = {&mNumPoints};
-  return LaunchP(FUNC_COMPUTE_FORCE, FX_STAGE_FORCE, args);
+  m_Done = FX_STAGE_FORCE;
+  return FX_OK;
 }
 
 int Particles::Advance() {
@@ -492,6 +507,18 @@ void Particles::AdvanceTime() {
 
 int Particles::Stage(int stage) {
   if (mNumPoints == 0) return fx::fail(FX_ESTATE, "stage: call Restart first");
+  // Each stage indexes with tables that the stages before it in the same step
+  // write (counts, offsets, sorted cells, the neighbour bitmask); a call out of
+  // that order is refused rather than run on stale tables.  Insert and advance
+  // read only per-particle data.
+  const bool ok = stage == FX_STAGE_PREFIX ? (m_Done == FX_STAGE_INSERT || m_Done == FX_STAGE_PREFIX)
+                  : stage == FX_STAGE_SORT ? m_Done == FX_STAGE_PREFIX
+                  : stage == FX_STAGE_PRESSURE ? m_Done >= FX_STAGE_SORT
+                  : stage == FX_STAGE_FORCE ? m_Done >= FX_STAGE_PRESSURE
+                                            : true;
+  if (!ok)
+    return fx::fail(FX_ESTATE, "stage %d: out of order (last stage run since the insert: %d); a step runs insert, "
+                    "prefix, sort, pressure, force, advance", stage, m_Done);
   switch (stage) {
     case FX_STAGE_INSERT: return InsertParticles();
     case FX_STAGE_PREFIX: return PrefixScanParticles();
@@ -501,6 +528,7 @@ int Particles::Stage(int stage) {
     case FX_STAGE_ADVANCE: {
       FX_TRY(Advance());
       AdvanceTime();
+      m_Done = FX_STAGE_ADVANCE;
       return FX_OK;
     }
     default: return fx::fail(FX_EINVAL, "stage: unknown stage %d", stage);
@@ -516,6 +544,7 @@ int Particles::Run() {
   FX_TRY(ComputeForce());
   FX_TRY(Advance());
   AdvanceTime();
+  m_Done = FX_STAGE_ADVANCE;
   return FX_OK;
 }
 
@@ -532,6 +561,7 @@ int Particles::CaptureGraph() {
   if (mNumPoints == 0) return fx::fail(FX_ESTATE, "CaptureGraph: call Restart first");
   FX_HIP(hipStreamSynchronize(m_Stream), "CaptureGraph");
   ReleaseGraph();
+  const int parity0 = m_Parity, done0 = m_Done;
   FX_HIP(hipStreamBeginCapture(m_Stream, hipStreamCaptureModeThreadLocal), "CaptureGraph");
   m_Capturing = true;
   int st = FX_OK;
@@ -546,11 +576,17 @@ int Particles::CaptureGraph() {
   m_Capturing = false;
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(m_Stream, &g);
-  if (st != FX_OK) {
+  if (st != FX_OK || e != hipSuccess) {
+    // nothing of a failed capture ran: restore the host view of the sets
     if (g) (void)hipGraphDestroy(g);
-    return st;
+    if (m_Parity != parity0) {
+      (void)m_Points.SwapGPU(&m_PointsTemp);
+      m_Parity = parity0;
+    }
+    m_Done = done0;
+    return st != FX_OK ? st : fx::hip_fail(e, "CaptureGraph", "hipStreamEndCapture");
   }
-  if (e != hipSuccess) return fx::hip_fail(e, "CaptureGraph", "hipStreamEndCapture");
+  m_Done = done0;  // recorded, not run
   m_Graph = g;
   m_GraphParity = m_Parity;
   FX_HIP(hipGraphInstantiate(&m_GraphExec, m_Graph, nullptr, nullptr, 0), "CaptureGraph");
@@ -567,6 +603,7 @@ int Particles::LaunchGraph(int nlaunch) {
     FX_HIP(hipGraphLaunch(m_GraphExec, m_Stream), "LaunchGraph");
     AdvanceTime();
     AdvanceTime();
+    m_Done = FX_STAGE_ADVANCE;
   }
   return FX_OK;
 }
@@ -761,7 +798,7 @@ int Particles::BenchSteps(int nsteps, bool use_graph, double* ms_total, double* 
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   const float time0 = m_Params.time;
-  const int parity0 = m_Parity;
+  const int parity0 = m_Parity, done0 = m_Done;
   if (st == FX_OK && use_graph) {
     if (hipStreamBeginCapture(m_Stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
       st = fx::fail(FX_EHIP, "bench: begin capture");
@@ -820,6 +857,7 @@ int Particles::BenchSteps(int nsteps, bool use_graph, double* ms_total, double* 
       (void)m_Points.SwapGPU(&m_PointsTemp);
       m_Parity = parity0;
     }
+    m_Done = done0;
   }
   if (ge) (void)hipGraphExecDestroy(ge);
   if (g) (void)hipGraphDestroy(g);
@@ -848,6 +886,7 @@ int Particles::TimeSteps(int nsteps, bool use_graph, double* ms_total) {
         if (hipGraphLaunch(m_GraphExec, m_Stream) != hipSuccess) st = fx::fail(FX_EHIP, "time_steps: graph launch");
         AdvanceTime();
         AdvanceTime();
+        m_Done = FX_STAGE_ADVANCE;
       }
       if (st == FX_OK && (nsteps & 1)) st = Run();
     } else {

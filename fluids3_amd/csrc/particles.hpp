@@ -161,6 +161,9 @@ class Particles {
   // device copy of both sets' pointer tables: [0] = set A, [1] = set B
   void* m_Tables = nullptr;
   int m_Parity = 0;  // which set m_Points holds
+  // the last stage (FX_STAGE_*) run on the device since the last insert, -1
+  // after a restart; Stage() refuses a stage whose input tables it has not made
+  int m_Done = -1;
   float* m_Clock = nullptr;
   void* m_Stats = nullptr;  // two uint64 counters of fxCountPairs
 

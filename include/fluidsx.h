@@ -144,7 +144,11 @@ int fx_load_points(fx_system* sys, const char* path);
 
 /* ---- stepping ---- */
 
-/* One per-function stage (FX_STAGE_*), enqueued on the stream. */
+/* One per-function stage (FX_STAGE_*), enqueued on the stream.  A stage whose
+ * input tables the current step has not made yet is refused (FX_ESTATE):
+ * prefix needs the insert, sort needs the prefix scan right after it,
+ * pressure needs the sort, force needs the pressure stage; insert and advance
+ * can always run. */
 int fx_stage(fx_system* sys, int32_t stage);
 /* Particles::Run n times (stages in order, then AdvanceTime). */
 int fx_run(fx_system* sys, int32_t nsteps);

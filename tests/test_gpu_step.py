@@ -145,6 +145,45 @@ def test_caller_runs_one_stage_on_the_cpu(ps):
     compare_state(got, want)
 
 
+def test_stages_out_of_order_are_refused(ps):
+    """A stage whose input tables the step has not made yet (offsets before the
+    scan, the sorted cells before the sort, the neighbour bitmask before the
+    density) is refused with an error instead of indexing stale tables; a graph
+    that was captured but not launched ran nothing."""
+    from fluids3_amd.particles import FluidsError
+
+    sc = scenes.dambreak_16k()
+    ps.Restart(sc)
+    for stage in (ps.PrefixScanParticles, ps.CountingSort, ps.ComputePressure, ps.ComputeForce):
+        with pytest.raises(FluidsError):
+            stage()
+    ps.CaptureGraph()
+    with pytest.raises(FluidsError):
+        ps.ComputePressure()
+    ps.InsertParticles()
+    with pytest.raises(FluidsError):
+        ps.CountingSort()  # the scan has not run
+    ps.PrefixScanParticles()
+    with pytest.raises(FluidsError):
+        ps.ComputeForce()  # nor the sort and the density
+    ps.CountingSort()
+    with pytest.raises(FluidsError):
+        ps.CountingSort()  # sorting the sorted set again
+    ps.ComputePressure()
+    ps.ComputeForce()
+    ps.Advance()
+    ps.ComputeForce()  # the tables still describe the current order
+    ps.LaunchGraph(1)
+    ps.ComputePressure()
+    # the refused calls changed nothing: the run equals a plain one
+    got = ps.state()
+    ps.Restart(sc)
+    ps.Run(3)
+    want = ps.state()
+    for k in ("pos", "vel", "veval", "id"):
+        assert bits_equal(got[k], want[k]), k
+
+
 def test_multi_step_bitexact_dambreak_16k(ps):
     sc = scenes.dambreak_16k()
     ps.Restart(sc)
