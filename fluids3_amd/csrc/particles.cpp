@@ -723,6 +723,7 @@ int Particles::ProfileStages(int nsteps, double* ms_out) {
 int Particles::SampleParticles(float* brick, const int* res, const float* bmin, const float* bmax, float scalar) {
   if (mNumPoints == 0) return fx::fail(FX_ESTATE, "sample_particles: call Restart first");
   if (m_Batch) return fx::fail(FX_ESTATE, "sample_particles: one scene only");
+  if (m_Done < FX_STAGE_SORT) return fx::fail(FX_ESTATE, "sample_particles: the cell table is made by a step's sort");
   if (brick == nullptr || res == nullptr || bmin == nullptr || bmax == nullptr)
     return fx::fail(FX_EINVAL, "sample_particles: null argument");
   if (res[0] <= 0 || res[1] <= 0 || res[2] <= 0 || (int64_t)res[0] * res[1] * res[2] > ((int64_t)1 << 31))
@@ -761,6 +762,7 @@ int Particles::EmitParticles(float frame, int emit) {
 
 int Particles::CountPairs(int64_t* cand, int64_t* pairs) {
   if (mNumPoints == 0 || m_Stats == nullptr) return fx::fail(FX_ESTATE, "count_pairs: call Restart first");
+  if (m_Done < FX_STAGE_SORT) return fx::fail(FX_ESTATE, "count_pairs: the cell table is made by a step's sort");
   int four = 4;
   unsigned zero = 0;
   void* st = m_Stats;

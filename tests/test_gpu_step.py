@@ -154,7 +154,8 @@ def test_stages_out_of_order_are_refused(ps):
 
     sc = scenes.dambreak_16k()
     ps.Restart(sc)
-    for stage in (ps.PrefixScanParticles, ps.CountingSort, ps.ComputePressure, ps.ComputeForce):
+    for stage in (ps.PrefixScanParticles, ps.CountingSort, ps.ComputePressure, ps.ComputeForce, ps.CountPairs,
+                  lambda: ps.SampleParticles((4, 4, 4), (-30.0, 0.0, -30.0), (30.0, 60.0, 30.0))):
         with pytest.raises(FluidsError):
             stage()
     ps.CaptureGraph()
