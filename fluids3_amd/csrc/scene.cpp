@@ -6,6 +6,7 @@
 //   Particles::UpdateParams      fluids5.0/src/particles.cpp:657-679
 //   Particles::AddPointsInVolume fluids5.0/src/particles.cpp:318-379
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 
 #include "fx_common.hpp"
@@ -163,9 +164,14 @@ int scene_fill(const fx_scene& sc, float* pos, uint32_t* clr, int32_t n) {
   if (n < 0) return fail(FX_EINVAL, "fx_scene_fill: n < 0");
   const float s = scene_spacing(sc);
   const fx_float3 mn = sc.init_min, mx = sc.init_max;
-  if (!(s > 0.0f)) return fail(FX_EINVAL, "fx_scene_fill: lattice spacing must be > 0");
-  const int cx = (int)std::ceil((mx.x - mn.x) / s);
-  const int cz = (int)std::ceil((mx.z - mn.z) / s);
+  if (!(s > 0.0f) || !std::isfinite(s)) return fail(FX_EINVAL, "fx_scene_fill: lattice spacing must be > 0");
+  if (!std::isfinite(mn.x) || !std::isfinite(mn.y) || !std::isfinite(mn.z) || !std::isfinite(mx.x) ||
+      !std::isfinite(mx.y) || !std::isfinite(mx.z))
+    return fail(FX_EINVAL, "fx_scene_fill: the fill volume must be finite");
+  const double ex = std::ceil(((double)mx.x - mn.x) / s), ez = std::ceil(((double)mx.z - mn.z) / s);
+  if (ex * (ez > 1.0 ? ez : 1.0) > (double)INT32_MAX)
+    return fail(FX_ERANGE, "fx_scene_fill: a layer of %.0f x %.0f lattice sites is too large", ex, ez);
+  const int cx = (int)ex, cz = (int)ez;
   const int cntx = cx > 0 ? cx : 1;
   const int cnt = cntx * (cz > 0 ? cz : 1);
   const float dx = mx.x - mn.x, dz = mx.z - mn.z;

@@ -87,6 +87,17 @@ def test_fill_is_deterministic_and_in_block():
     assert np.all((clr >> 24) == 255)
 
 
+def test_fill_rejects_bad_volumes():
+    sc = scenes.dambreak_16k()
+    sc.spacing = 1e-9  # 6e10 sites per layer
+    with pytest.raises(FluidsError, match="too large"):
+        fill_points(sc, 10)
+    sc = scenes.dambreak_16k()
+    sc.init_max.x = float("nan")
+    with pytest.raises(FluidsError, match="finite"):
+        fill_points(sc, 10)
+
+
 def test_batch_scene_counts():
     sc = scenes.batch_scenes()
     n = [s.num_points for s in sc]
