@@ -29,6 +29,7 @@ vs_baseline = value / 6.4e8, RTX 4090, README.md:12-19).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -92,7 +93,9 @@ def main():
                          "advance); empty = the library's defaults")
     ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly")
     ap.add_argument("--stats-every", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="oracle steps timed for the CPU baseline; 0 = as many as fill --cpu-seconds (at most 50)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = every core this process may use")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="batch64: skip the one-scene-at-a-time comparison")
@@ -255,11 +258,19 @@ def main():
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
         st = oracle.make_state(p, start_state["pos"], vel=start_state["vel"], veval=start_state["veval"],
                                clr=start_state["clr"], ids=start_state["id"], time=start_state["time"])
-        sec = oracle.time_steps(p, st, args.cpu_steps, threads=threads)
-        cpu = {"value": round(N * args.cpu_steps / sec, 1), "unit": "particle-steps/s", "cores": threads,
+        if args.cpu_steps > 0:
+            steps = args.cpu_steps
+            sec = oracle.time_steps(p, st, steps, threads=threads)
+        else:  # one step, then as many more as fill the time asked for
+            sec = oracle.time_steps(p, st, 1, threads=threads)
+            more = max(0, min(49, int(math.ceil((args.cpu_seconds - sec) / max(sec, 1e-6)))))
+            steps = 1 + more
+            if more:
+                sec += oracle.time_steps(p, st, more, threads=threads)
+        cpu = {"value": round(N * steps / sec, 1), "unit": "particle-steps/s", "cores": threads,
                "kind": "port",
-               "sample": f"{args.cpu_steps} FP32 oracle step(s) of the same {N:,}-particle state at the start "
-                         f"of the timed window on {threads} host threads ({sec:.2f} s)"}
+               "sample": f"{steps} consecutive FP32 oracle step(s) of the same {N:,}-particle {args.workload} state "
+                         f"from the start of the timed window on {threads} host threads ({sec:.2f} s)"}
 
     out = {
         # the headline workload carries the forward+adjoint ratio in the same line ("adjoint")
