@@ -171,9 +171,10 @@ class Particles {
   hipGraphExec_t m_GraphExec = nullptr;
   bool m_Capturing = false;
   int m_GraphParity = 0;
-  // threads per block of each stage's per-particle kernels; the force stage
-  // runs one independent wave per block
-  int m_StageThreads[FX_NUM_STAGES] = {256, 256, 256, 256, 64, 256};
+  // threads per block of each stage's per-particle kernels (insert, prefix,
+  // sort, pressure, force, advance): the sizes fx_autotune keeps choosing on
+  // the dam-break and beach scenes at 4M particles
+  int m_StageThreads[FX_NUM_STAGES] = {128, 256, 512, 256, 256, 64};
 
   // batch of scenes (m_Batch); a single scene uses the reference-named kernels
   bool m_Batch = false;

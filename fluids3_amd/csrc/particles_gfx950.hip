@@ -293,19 +293,53 @@ __device__ __forceinline__ void sort_index_impl(const FParams_t& P, const Span S
 // Cells above FX_RANK_CAP members keep the atomic order.  The sort moves the
 // state that outlives a step (pos, vel, veval, clr, id) and the fresh gcell
 // and gndx; press and force are recomputed by the next two stages.
+//
+// Cells of more than FX_RANK_WAVE members are ranked by the wave together:
+// the particles of a wave are consecutive, so the members of a cell below
+// lane l's index are the members below the wave's first index w0 (counted
+// once per distinct cell, 64 members per load) plus the wave's lanes below l
+// in the same cell.  Smaller cells are ranked per lane.
+#ifndef FX_RANK_WAVE
+#define FX_RANK_WAVE 32
+#endif
+
 __device__ __forceinline__ void sort_full_impl(const FParams_t& P, const Span S) {
+  const int lane = (int)(threadIdx.x & 63u);
   const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
-  if (i >= S.p1) return;
-  const int c = pbuf<const int>(FPntTmp, FGCELL)[i];
-  const int cell = (c == FX_GRID_UNDEF) ? S.c0 + P.gridTotal : c;
+  const bool act = i < S.p1;  // active lanes are a prefix of the wave
   const uint* off = pbuf<const uint>(FAccel, AGRIDOFF);
   const uint* cnt = pbuf<const uint>(FAccel, AGRIDCNT);
   const uint* grid = pbuf<const uint>(FAccel, AGRID);
-  const uint o = off[cell];
-  const uint n = cnt[cell];
   const uint* tscan = pbuf<const uint>(FAccel, ATAILSCAN);
-  int rank;
-  if (c == FX_GRID_UNDEF && tscan != nullptr) {
+  const int c = act ? pbuf<const int>(FPntTmp, FGCELL)[i] : FX_GRID_UNDEF;
+  const int cell = (c == FX_GRID_UNDEF) ? S.c0 + P.gridTotal : c;
+  const uint o = act ? off[cell] : 0u;
+  const uint n = act ? cnt[cell] : 0u;
+  const bool wide = act && c != FX_GRID_UNDEF && n > FX_RANK_WAVE && n <= FX_RANK_CAP;
+  int rank = 0;
+  {
+    const uint w0 = (uint)(i - lane);
+    unsigned long long pending = __ballot(wide);
+    while (pending != 0ull) {  // one distinct dense cell per pass (wave-uniform)
+      const int lead = __ffsll((long long)pending) - 1;
+      const int cl = __shfl(cell, lead, 64);
+      const uint ol = (uint)__shfl((int)o, lead, 64), nl = (uint)__shfl((int)n, lead, 64);
+      int below = 0;
+      for (uint k0 = 0; k0 < nl; k0 += 64) {
+        const uint k = k0 + (uint)lane;
+        const uint m = k < nl ? grid[ol + k] : 0xffffffffu;
+        below += __popcll(__ballot(m < w0));
+      }
+      const bool mine = wide && cell == cl;
+      const unsigned long long same = __ballot(mine);
+      if (mine) rank = below + __popcll(same & ((1ull << lane) - 1ull));
+      pending &= ~same;
+    }
+  }
+  if (!act) return;
+  if (wide) {
+    // ranked above
+  } else if (c == FX_GRID_UNDEF && tscan != nullptr) {
     // tail bucket: the number of the scene's out-of-range particles before i
     rank = (int)(tscan[i] - tscan[S.p0]);
   } else if (n <= FX_RANK_CAP) {
