@@ -521,17 +521,39 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
         const float4 q = tile[k];
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        bool pair = dsq < rd2 && dsq > 0.0f;
-        if (decltype(checked)::value) pair = pair && s < len;
-        const float tt = pair ? (rd2 - dsq) * d2 : 0.0f;
-        sum = fmaf(tt * tt, tt, sum);  // + 0 when not a pair: sum unchanged
-        acc = acc + acc + (pair ? 1u : 0u);
+        if (decltype(checked)::value) {
+          const bool pair = dsq < rd2 && dsq > 0.0f && s < len;
+          const float tt = pair ? (rd2 - dsq) * d2 : 0.0f;
+          sum = fmaf(tt * tt, tt, sum);  // + 0 when not a pair: sum unchanged
+          acc = acc + acc + (pair ? 1u : 0u);
+        } else {
+          // The same three steps with the wave's pair mask m used as it is:
+          // the term selected by m (v_cndmask) and the bit shifted in as the
+          // carry of acc + acc (v_addc), one instruction each where the
+          // compiler spends two on the bit.  One ballot per compare, so that
+          // m is the compares' own mask.
+          const unsigned long long m = __builtin_amdgcn_ballot_w64(dsq < rd2) & __builtin_amdgcn_ballot_w64(dsq > 0.0f);
+          const float t0 = (rd2 - dsq) * d2;
+          float tt;
+          __asm__("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(tt) : "v"(t0), "s"(m));
+          sum = fmaf(tt * tt, tt, sum);  // + 0 when not a pair: sum unchanged
+          unsigned long long carry;
+          uint acc2;
+          __asm__("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(acc2), "=s"(carry) : "v"(acc), "s"(m));
+          acc = acc2;
+        }
       };
       for (int s0 = 0; s0 < lmax; s0 += 32) {
         const int n = min(32, lmax - s0);
         const int nu = max(0, min(n, lmin - s0));
         int u = 0;
-#pragma unroll 4
+        // unrolled by hand: the asm in cand keeps the compiler from unrolling
+        for (; u + 4 <= nu; u += 4) {
+          cand(s0 + u, std::false_type());
+          cand(s0 + u + 1, std::false_type());
+          cand(s0 + u + 2, std::false_type());
+          cand(s0 + u + 3, std::false_type());
+        }
         for (; u < nu; u++) cand(s0 + u, std::false_type());
         for (; u < n; u++) cand(s0 + u, std::true_type());
         if (rec && (s0 >> 5) < nw) bits[(size_t)(W + (s0 >> 5)) * NN + i] = __builtin_bitreverse32(acc << (32 - n));
