@@ -186,13 +186,14 @@ def test_stages_out_of_order_are_refused(ps):
 
 
 def test_multi_step_bitexact_dambreak_16k(ps):
+    """The 16,384-particle dam break for its full 100 steps, bit for bit."""
     sc = scenes.dambreak_16k()
     ps.Restart(sc)
     p = ps.params
     s0 = ps.state()
-    ps.Run(25)
+    ps.Run(100)
     g = ps.state()
-    ref = oracle.run(p, fresh_oracle(p, s0), 25, threads=THREADS)
+    ref = oracle.run(p, fresh_oracle(p, s0), 100, threads=THREADS)
     compare_state(g, ref)
 
 
@@ -297,6 +298,22 @@ def test_dense_cells_without_bitmask(ps, threads):
     # the first step's sorted state: some particle has more candidates than the bitmask holds
     srt = oracle.run(p, fresh_oracle(p, s0), 1, oracle.STAGE_INSERT | oracle.STAGE_SCAN | oracle.STAGE_SORT)
     assert oracle.pairs_grid(p, srt)[0].max() > 32 * abi.NBR_WORDS
+
+
+def test_compressed_block_trajectory(ps):
+    """64,000 particles at the 4M dam break's lattice spacing (0.7, about 45
+    per cell, up to ~60 as the block collapses), 30 steps bit for bit: cells of
+    more than 32 members, which the sort ranks wave-cooperatively, long density
+    rows and deep force queues, over a trajectory."""
+    sc = scenes.dam_break(64000, (-60, 0, -30), (60, 60, 30), (-54, 4, -14), (-26, 32, 14))
+    ps.Restart(sc)
+    p = ps.params
+    s0 = ps.state()
+    ps.Run(30)
+    g = ps.state()
+    ref = oracle.run(p, fresh_oracle(p, s0), 30, threads=THREADS)
+    compare_state(g, ref)
+    assert g["cnt"][:-1].max() > 32
 
 
 @pytest.mark.parametrize("n", [1, 2, 63, 65, 1000, 4097])
