@@ -31,7 +31,7 @@ __constant__ cuDataX FPntTmp;
 __constant__ cuDataX FAccel;
 
 #define SCAN_BLOCKSIZE 512  // threads per scan block; each covers 1024 cells
-#define FX_RANK_CAP 2048    // cells with more members keep atomic order
+#define FX_RANK_CAP 65536   // cells with more members keep atomic order
 
 // ---------------------------------------------------------------- helpers
 

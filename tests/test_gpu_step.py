@@ -377,6 +377,27 @@ def test_large_tail_bucket_is_stable(ps):
         assert bits_equal(g[k], again[k]), k
 
 
+def test_crowded_cell_is_stable(ps):
+    """3,000 particles packed into one cell (more than the 2,048 members a cell
+    of the per-lane and earlier ranking covered): the sort ranks the cell
+    wave-cooperatively, stable by particle index, and two steps equal the
+    oracle bit for bit."""
+    sc = scenes.dambreak_16k()
+    ps.Restart(sc)
+    p = ps.params
+    pos = ps.state()["pos"].copy()
+    rng = np.random.default_rng(12)
+    lo = np.array([-22.3, 10.2, -9.8], np.float32)  # inside cell (5, 6, 10) of the 2.5-unit grid
+    pos[:3000] = (lo + rng.uniform(0.0, 2.1, (3000, 3))).astype(np.float32)
+    ps.Commit(abi.FPOS, pos)
+    s0 = ps.state()
+    first = oracle.run(p, fresh_oracle(p, s0), 1, oracle.STAGE_INSERT)
+    assert first["cnt"][:-1].max() >= 3000  # what the first step's sort ranks
+    ps.Run(2)
+    g = ps.state()
+    compare_state(g, oracle.run(p, fresh_oracle(p, s0), 2, threads=THREADS))
+
+
 @pytest.mark.parametrize("n", [1, 2, 1023, 1024, 1025, 2047, 2048, 4097, 1024 * 1024 - 1, 1024 * 1024,
                                1024 * 1024 + 1, 3 * 1024 * 1024 + 7])
 def test_scan_lengths(ps, n):
