@@ -26,9 +26,20 @@
 
 namespace {
 
-enum { ADJ_LOSS, ADJ_ADVANCE, ADJ_FORCE, ADJ_DENSITY, ADJ_REDUCE, ADJ_FRAME_IN, ADJ_FRAME_OUT, ADJ_UNFRAME, ADJ_NFUNC };
-const char* kAdjNames[ADJ_NFUNC] = {"adjLoss",    "adjAdvance",  "adjForce",  "adjDensity",
-                                    "adjReduce",  "adjFrameIn",  "adjFrameOut", "adjUnframe"};
+enum {
+  ADJ_LOSS, ADJ_ADVANCE, ADJ_FORCE, ADJ_DENSITY, ADJ_REDUCE, ADJ_FRAME_IN, ADJ_FRAME_OUT, ADJ_UNFRAME, ADJ_TAPE,
+  ADJ_NFUNC
+};
+const char* kAdjNames[ADJ_NFUNC] = {"adjLoss",   "adjAdvance",  "adjForce",   "adjDensity",   "adjReduce",
+                                    "adjFrameIn", "adjFrameOut", "adjUnframe", "adjTapeRecord"};
+
+// source arrays of one tape slot (TapeSrc in adjoint_gfx950.hip)
+struct TapeSrc {
+  const void* v3[4];
+  const void* s1[3];
+  const void* cnt;
+  const void* off;
+};
 const int kAdjThreads = 64;   // ADJ_THREADS: one wave per block
 const int kRedThreads = 256;  // RED_THREADS
 
@@ -290,17 +301,16 @@ int Particles::AdjStep(int record_slot) {
   if (record_slot >= 0) {
     FX_HIP(hipMemcpyAsync(m_Accel.SymbolAddress(), tabs, sizeof(cuDataX), hipMemcpyDeviceToDevice, m_Stream),
            "tape");
+    // pos, vel, veval, force, rho, id, gcell, then the cell counts and offsets: one kernel
     const size_t n = (size_t)m_AdjN;
     char* t = m_AdjTape + (kTapeBytes * n + 8 * (size_t)m_AdjCells) * (size_t)record_slot;
-    const int ids[7] = {FPOS, FVEL, FVEVAL, FFORCE, FPRESS, FID, FGCELL};
-    const size_t sz[7] = {12, 12, 12, 12, 4, 4, 4};
-    for (int b = 0; b < 7; b++) {
-      FX_HIP(hipMemcpyAsync(t, m_Points.gpu(ids[b]), sz[b] * n, hipMemcpyDeviceToDevice, m_Stream), "tape");
-      t += sz[b] * n;
-    }
-    const size_t cb = 4 * (size_t)m_AdjCells;
-    FX_HIP(hipMemcpyAsync(t, m_Accel.gpu(AGRIDCNT), cb, hipMemcpyDeviceToDevice, m_Stream), "tape");
-    FX_HIP(hipMemcpyAsync(t + cb, m_Accel.gpu(AGRIDOFF), cb, hipMemcpyDeviceToDevice, m_Stream), "tape");
+    TapeSrc src = {{m_Points.gpu(FPOS), m_Points.gpu(FVEL), m_Points.gpu(FVEVAL), m_Points.gpu(FFORCE)},
+                   {m_Points.gpu(FPRESS), m_Points.gpu(FID), m_Points.gpu(FGCELL)},
+                   m_Accel.gpu(AGRIDCNT),
+                   m_Accel.gpu(AGRIDOFF)};
+    int nn = m_AdjN, cells = m_AdjCells;
+    void* a[] = {&src, &t, &nn, &cells};
+    FX_TRY(launch_n(m_AdjFunc[ADJ_TAPE], kAdjNames[ADJ_TAPE], m_Stream, nn > cells ? nn : cells, a));
     m_AdjTapeTime[record_slot] = m_Params.time;
   }
   FX_TRY(Advance());

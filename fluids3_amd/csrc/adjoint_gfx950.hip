@@ -194,6 +194,41 @@ extern "C" __global__ void adjUnframe(const fx_float3* g, fx_float3* out, const 
   for (int v = 0; v < 3; v++) out[(size_t)v * n + id] = g[(size_t)v * n + a];
 }
 
+// ------------------------------------------------------------------- tape
+// One recorded step's tape slot, written in one pass: pos, vel, veval, force
+// (12 B each), rho, id, gcell (4 B each) of the n particles, then the cell
+// counts and offsets of the cells-entry table, each array contiguous in that
+// order.  Thread a copies particle a and table entry a.
+struct TapeSrc {
+  const fx_float3* v3[4];  // pos, vel, veval, force
+  const uint* s1[3];       // rho (as bits), id, gcell
+  const uint* cnt;
+  const uint* off;
+};
+
+extern "C" __global__ void adjTapeRecord(TapeSrc S, char* dst, int n, int cells) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = (size_t)n;
+  if (a < n) {
+    fx_float3 v[4];
+    uint w[3];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = S.v3[k][a];
+#pragma unroll
+    for (int k = 0; k < 3; k++) w[k] = S.s1[k][a];
+#pragma unroll
+    for (int k = 0; k < 4; k++) reinterpret_cast<fx_float3*>(dst + 12 * N * k)[a] = v[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) reinterpret_cast<uint*>(dst + 48 * N + 4 * N * k)[a] = w[k];
+  }
+  if (a < cells) {
+    const uint c = S.cnt[a], o = S.off[a];
+    uint* t = reinterpret_cast<uint*>(dst + 60 * N);
+    t[a] = c;
+    t[(size_t)cells + a] = o;
+  }
+}
+
 // ---------------------------------------------------------------- advance
 extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) {
   const Slot S = slot_of(A);

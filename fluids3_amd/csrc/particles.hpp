@@ -192,7 +192,7 @@ class Particles {
   hipStream_t m_AdjStream = nullptr;     // the backward's adjoint kernels (default priority)
   hipEvent_t m_AdjEv[5] = {};            // per tape buffer: recomputed [0,1], adjoint done [2,3]; join [4]
   int m_AdjBufs = 1, m_AdjTapeSteps = 0;  // tape buffers (2 when segments are recomputed) and slots in each
-  hipFunction_t m_AdjFunc[8] = {};
+  hipFunction_t m_AdjFunc[9] = {};
   int m_AdjSteps = 0, m_AdjEvery = 0, m_AdjSegs = 0, m_AdjN = 0, m_AdjCells = 0;
   bool m_AdjTapeValid = false, m_AdjLossDone = false;
   int64_t m_AdjBytes = 0;
