@@ -28,10 +28,16 @@ namespace {
 
 enum {
   ADJ_LOSS, ADJ_ADVANCE, ADJ_FORCE, ADJ_DENSITY, ADJ_REDUCE, ADJ_FRAME_IN, ADJ_FRAME_OUT, ADJ_UNFRAME, ADJ_TAPE,
-  ADJ_NFUNC
+  ADJ_CKPT, ADJ_NFUNC
 };
-const char* kAdjNames[ADJ_NFUNC] = {"adjLoss",   "adjAdvance",  "adjForce",   "adjDensity",   "adjReduce",
-                                    "adjFrameIn", "adjFrameOut", "adjUnframe", "adjTapeRecord"};
+const char* kAdjNames[ADJ_NFUNC] = {"adjLoss",    "adjAdvance",  "adjForce",   "adjDensity",    "adjReduce",
+                                    "adjFrameIn", "adjFrameOut", "adjUnframe", "adjTapeRecord", "adjCheckpoint"};
+
+// particle buffers of a checkpoint (CkptPtr in adjoint_gfx950.hip)
+struct CkptPtr {
+  void* v3[3];
+  void* s1[2];
+};
 
 // source arrays of one tape slot (TapeSrc in adjoint_gfx950.hip)
 struct TapeSrc {
@@ -260,28 +266,23 @@ int Particles::AdjLaunch(int fid, void* args, hipStream_t st) {
   return FX_OK;
 }
 
+// pos, vel, veval, clr, id of the current set to (save) or from checkpoint k, one kernel
+int Particles::AdjCheckpointCopy(int k, int save) {
+  char* c = m_AdjCkpt + kCkptBytes * (size_t)m_AdjN * (size_t)k;
+  CkptPtr p = {{m_Points.gpu(FPOS), m_Points.gpu(FVEL), m_Points.gpu(FVEVAL)}, {m_Points.gpu(FCLR), m_Points.gpu(FID)}};
+  int nn = m_AdjN;
+  void* a[] = {&p, &c, &nn, &save};
+  return launch_n(m_AdjFunc[ADJ_CKPT], kAdjNames[ADJ_CKPT], m_Stream, nn, a);
+}
+
 int Particles::AdjSaveCheckpoint(int k) {
-  const size_t n = (size_t)m_AdjN;
-  char* c = m_AdjCkpt + kCkptBytes * n * (size_t)k;
-  const int ids[5] = {FPOS, FVEL, FVEVAL, FCLR, FID};
-  const size_t sz[5] = {12, 12, 12, 4, 4};
-  for (int b = 0; b < 5; b++) {
-    FX_HIP(hipMemcpyAsync(c, m_Points.gpu(ids[b]), sz[b] * n, hipMemcpyDeviceToDevice, m_Stream), "checkpoint");
-    c += sz[b] * n;
-  }
+  FX_TRY(AdjCheckpointCopy(k, 1));
   m_AdjCkptTime[k] = m_Params.time;
   return FX_OK;
 }
 
 int Particles::AdjRestoreCheckpoint(int k) {
-  const size_t n = (size_t)m_AdjN;
-  const char* c = m_AdjCkpt + kCkptBytes * n * (size_t)k;
-  const int ids[5] = {FPOS, FVEL, FVEVAL, FCLR, FID};
-  const size_t sz[5] = {12, 12, 12, 4, 4};
-  for (int b = 0; b < 5; b++) {
-    FX_HIP(hipMemcpyAsync(m_Points.gpu(ids[b]), c, sz[b] * n, hipMemcpyDeviceToDevice, m_Stream), "restore");
-    c += sz[b] * n;
-  }
+  FX_TRY(AdjCheckpointCopy(k, 0));
   m_Params.time = m_AdjCkptTime[k];
   return FX_OK;
 }

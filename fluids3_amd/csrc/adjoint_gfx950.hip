@@ -229,6 +229,36 @@ extern "C" __global__ void adjTapeRecord(TapeSrc S, char* dst, int n, int cells)
   }
 }
 
+// A checkpoint: pos, vel, veval (12 B each), clr, id (4 B each) of the n
+// particles, contiguous in that order in ckpt.  save != 0 copies the state
+// into the checkpoint, save == 0 back into the particle buffers.
+struct CkptPtr {
+  fx_float3* v3[3];  // pos, vel, veval
+  uint* s1[2];       // clr, id
+};
+
+extern "C" __global__ void adjCheckpoint(CkptPtr S, char* ckpt, int n, int save) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const size_t N = (size_t)n;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    fx_float3* c = reinterpret_cast<fx_float3*>(ckpt + 12 * N * k) + a;
+    if (save)
+      *c = S.v3[k][a];
+    else
+      S.v3[k][a] = *c;
+  }
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    uint* c = reinterpret_cast<uint*>(ckpt + 36 * N + 4 * N * k) + a;
+    if (save)
+      *c = S.s1[k][a];
+    else
+      S.s1[k][a] = *c;
+  }
+}
+
 // ---------------------------------------------------------------- advance
 extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) {
   const Slot S = slot_of(A);

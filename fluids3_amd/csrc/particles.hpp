@@ -140,6 +140,7 @@ class Particles {
   void AdjRelease();
   int AdjTables();
   int AdjStep(int record_slot);
+  int AdjCheckpointCopy(int k, int save);
   int AdjSaveCheckpoint(int k);
   int AdjRestoreCheckpoint(int k);
   int AdjLaunch(int fid, void* args, hipStream_t st);
@@ -192,7 +193,7 @@ class Particles {
   hipStream_t m_AdjStream = nullptr;     // the backward's adjoint kernels (default priority)
   hipEvent_t m_AdjEv[5] = {};            // per tape buffer: recomputed [0,1], adjoint done [2,3]; join [4]
   int m_AdjBufs = 1, m_AdjTapeSteps = 0;  // tape buffers (2 when segments are recomputed) and slots in each
-  hipFunction_t m_AdjFunc[9] = {};
+  hipFunction_t m_AdjFunc[10] = {};
   int m_AdjSteps = 0, m_AdjEvery = 0, m_AdjSegs = 0, m_AdjN = 0, m_AdjCells = 0;
   bool m_AdjTapeValid = false, m_AdjLossDone = false;
   int64_t m_AdjBytes = 0;
