@@ -515,7 +515,7 @@ __device__ __forceinline__ void ring_walk(const FParams_t& P, const AdjArgs& A, 
 #define AW_QD 8   // and in the density adjoint
 #endif
 #ifndef AW_STEP
-#define AW_STEP 2  // pairs taken per drain step
+#define AW_STEP 3  // pairs taken per drain step
 #endif
 #ifndef AW_FORCE_ATTR
 #define AW_FORCE_ATTR
@@ -565,10 +565,13 @@ __device__ __forceinline__ void queue_walk(const FParams_t& P, const AdjArgs& A,
     for (; e + AW_STEP <= f; e += AW_STEP) {
       uint j[AW_STEP];
       bool ok[AW_STEP];
+      typename Op::Rec rc[AW_STEP];
 #pragma unroll
       for (int u = 0; u < AW_STEP; u++) ok[u] = take(j[u]);
 #pragma unroll
-      for (int u = 0; u < AW_STEP; u++) op.eval(j[u], ok[u]);
+      for (int u = 0; u < AW_STEP; u++) rc[u] = op.load(j[u]);  // every gather of the step first
+#pragma unroll
+      for (int u = 0; u < AW_STEP; u++) op.apply(rc[u], ok[u]);
     }
     for (; e < f; e++) {
       uint j;
@@ -639,8 +642,16 @@ struct ForceAdj {
   float kp, stiff, vterm, lap, h, rest, d2;
   float gpx, gpy, gpz, gex, gey, gez, gr, ak, av;
 
-  __device__ __forceinline__ void eval(uint j, bool ok) {
-    const float4 a = rec[3 * (size_t)j], v = rec[3 * (size_t)j + 1], q = rec[3 * (size_t)j + 2];
+  // the pair's records (load), then its terms added in order (apply)
+  struct Rec {
+    float4 a, v, q;
+  };
+  __device__ __forceinline__ Rec load(uint j) const {
+    return Rec{rec[3 * (size_t)j], rec[3 * (size_t)j + 1], rec[3 * (size_t)j + 2]};
+  }
+  __device__ __forceinline__ void eval(uint j, bool ok) { apply(load(j), ok); }
+  __device__ __forceinline__ void apply(const Rec& r, bool ok) {
+    const float4 a = r.a, v = r.v, q = r.q;
     const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
     const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const float rr = sqrtf(dsq * d2);
@@ -753,8 +764,13 @@ struct DensityAdj {
   float px, py, pz, gi, d2, rd2, k6;
   float gpx, gpy, gpz;
 
-  __device__ __forceinline__ void eval(uint j, bool ok) {
-    const float4 a = rec[j];
+  struct Rec {
+    float4 a;
+  };
+  __device__ __forceinline__ Rec load(uint j) const { return Rec{rec[j]}; }
+  __device__ __forceinline__ void eval(uint j, bool ok) { apply(load(j), ok); }
+  __device__ __forceinline__ void apply(const Rec& r, bool ok) {
+    const float4 a = r.a;
     const float dx = px - a.x, dy = py - a.y, dz = pz - a.z;
     const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     const float t = (rd2 - dsq) * d2;
