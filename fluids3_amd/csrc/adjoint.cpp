@@ -14,7 +14,7 @@
 // adjoint of segment s runs.  For the length of a rollout the particles are
 // numbered by their slot at its start (see adjFrameIn).  Memory: (segments
 // + 1) x 44 B x N of checkpoints plus every x (60 B x N + 8 B x cells +
-// FX_NBR_WORDS x 4 B x N of neighbour bitmask) of tape, twice when there is
+// W x 4 B x N of neighbour bitmask, W sized from the densest stencil) of tape, twice when there is
 // more than one segment and the second buffer fits in 60 % of the free
 // device memory.
 #include <cstdlib>
@@ -78,12 +78,13 @@ struct AdjArgs {
   int nbits;
   float time;
   int accumulate;
+  int ncap;
 };
 
 const size_t kCkptBytes = 44;   // pos, vel, veval (12 B each), clr, id (4 B each)
 const size_t kTapeBytes = 60;   // pos, vel, veval, force (12 B each), rho, id, gcell (4 B each)
 // plus, per recorded step, the density kernel's neighbour bitmask
-// (FX_NBR_WORDS words per particle), written there directly by the forward
+// (m_AdjWords words per particle), written there directly by the forward
 
 // one thread per particle, 256 per block
 int launch_n(hipFunction_t f, const char* name, hipStream_t st, int n, void** args) {
@@ -99,6 +100,8 @@ void Particles::AdjRelease() {
   m_AdjAlloc.clear();
   m_AdjCkpt = m_AdjTape = nullptr;
   m_AdjBits = nullptr;
+  m_AdjWordCaps = nullptr;
+  m_AdjWords = FX_NBR_WORDS;
   m_AdjArec = m_AdjDrec = m_AdjTabs = nullptr;
   m_AdjVec = nullptr;
   m_AdjId0 = nullptr;
@@ -132,12 +135,29 @@ int Particles::AdjConfigure(const char* path, int nsteps, int every) {
   AdjRelease();
   const int n = mNumPoints;
   const size_t tape_step = kTapeBytes * (size_t)n + 8 * (size_t)m_ScanCells;
+  // Bitmask words per particle a tape slot holds: twice the densest stencil
+  // of the current state (counts and offsets from an insert and scan, which
+  // reorder nothing), at least 32 and at most FX_NBR_WORDS, a multiple of 4;
+  // FX_ADJOINT_TAPE_WORDS in the environment sets it.  A particle whose
+  // stencil needs more during the rollout records no bits and is walked by
+  // testing its candidates, forward and reverse.
+  FX_TRY(InsertParticles());
+  FX_TRY(PrefixScanParticles());
+  int maxw = 0;
+  FX_TRY(MaxNbrWords(&maxw));
+  int words = 2 * maxw;
+  words = words < 32 ? 32 : (words > FX_NBR_WORDS ? FX_NBR_WORDS : words);
+  words = (words + 3) / 4 * 4;
+  if (const char* w = getenv("FX_ADJOINT_TAPE_WORDS")) {
+    const int v = atoi(w);
+    if (v >= 1 && v <= FX_NBR_WORDS) words = v;
+  }
   // memory budget: 60 % of the free device memory
   size_t free_b = 0, total_b = 0;
   FX_HIP(hipMemGetInfo(&free_b, &total_b), "adjoint: querying device memory");
   const double budget = 0.6 * (double)free_b;
   const double fixed = (double)(sizeof(float) * 25 + 4 + 48 + 16) * n;
-  const double per_step = (double)tape_step + 4.0 * FX_NBR_WORDS * n;
+  const double per_step = (double)tape_step + 4.0 * words * n;
   if (every <= 0) {
     // The backward recomputes every segment but the last, (segments - 1) x
     // every steps: take the fewest segments whose tape (with its bitmasks)
@@ -183,8 +203,15 @@ int Particles::AdjConfigure(const char* path, int nsteps, int every) {
   m_AdjVec = static_cast<float*>(p);
   FX_TRY(alloc(sizeof(uint32_t) * (size_t)n, &p));
   m_AdjId0 = static_cast<uint32_t*>(p);
-  FX_TRY(alloc(sizeof(uint32_t) * FX_NBR_WORDS * (size_t)n * (size_t)tape_steps, &p));
+  FX_TRY(alloc(sizeof(uint32_t) * (size_t)words * (size_t)n * (size_t)tape_steps, &p));
   m_AdjBits = static_cast<char*>(p);
+  FX_TRY(alloc(2 * sizeof(int), &p));
+  m_AdjWordCaps = static_cast<int*>(p);
+  {
+    const int caps[2] = {FX_NBR_WORDS, words};
+    FX_HIP(hipMemcpyHtoD(m_AdjWordCaps, const_cast<int*>(caps), sizeof(caps)), "adjoint");
+  }
+  m_AdjWords = words;
   FX_TRY(alloc(48 * (size_t)n, &m_AdjArec));
   FX_TRY(alloc(16 * (size_t)n, &m_AdjDrec));
   FX_TRY(alloc(sizeof(cuDataX) * (size_t)(tape_steps + 1), &m_AdjTabs));
@@ -195,7 +222,7 @@ int Particles::AdjConfigure(const char* path, int nsteps, int every) {
     for (int k = 0; k < tape_steps; k++) {
       tabs[k + 1] = tabs[0];
       tabs[k + 1].mbuf[ANBRBITS] =
-          (uint64_t)(uintptr_t)(m_AdjBits + sizeof(uint32_t) * FX_NBR_WORDS * (size_t)n * (size_t)k);
+          (uint64_t)(uintptr_t)(m_AdjBits + sizeof(uint32_t) * (size_t)words * (size_t)n * (size_t)k);
     }
     FX_HIP(hipMemcpyHtoD(m_AdjTabs, tabs.data(), sizeof(cuDataX) * tabs.size()), "adjoint");
   }
@@ -294,14 +321,17 @@ int Particles::AdjStep(int record_slot) {
   FX_TRY(PrefixScanParticles());
   FX_TRY(CountingSort());
   char* tabs = static_cast<char*>(m_AdjTabs);
-  if (record_slot >= 0)  // the density kernel writes this step's bitmask into the tape
+  if (record_slot >= 0) {  // the density kernel writes this step's bitmask into the tape, m_AdjWords a particle
+    FX_HIP(hipMemcpyAsync(m_NbrWordsSym, m_AdjWordCaps + 1, sizeof(int), hipMemcpyDeviceToDevice, m_Stream), "tape");
     FX_HIP(hipMemcpyAsync(m_Accel.SymbolAddress(), tabs + sizeof(cuDataX) * (size_t)(record_slot + 1),
                           sizeof(cuDataX), hipMemcpyDeviceToDevice, m_Stream), "tape");
+  }
   FX_TRY(ComputePressure());
   FX_TRY(ComputeForce());
   if (record_slot >= 0) {
     FX_HIP(hipMemcpyAsync(m_Accel.SymbolAddress(), tabs, sizeof(cuDataX), hipMemcpyDeviceToDevice, m_Stream),
            "tape");
+    FX_HIP(hipMemcpyAsync(m_NbrWordsSym, m_AdjWordCaps, sizeof(int), hipMemcpyDeviceToDevice, m_Stream), "tape");
     // pos, vel, veval, force, rho, id, gcell, then the cell counts and offsets: one kernel
     const size_t n = (size_t)m_AdjN;
     char* t = m_AdjTape + (kTapeBytes * n + 8 * (size_t)m_AdjCells) * (size_t)record_slot;
@@ -460,8 +490,9 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
       A.partial = m_AdjPartial;
       A.arec = m_AdjArec;
       A.drec = m_AdjDrec;
-      A.bits = m_AdjBits + sizeof(uint32_t) * FX_NBR_WORDS * n * (size_t)k;
+      A.bits = m_AdjBits + sizeof(uint32_t) * (size_t)m_AdjWords * n * (size_t)k;
       A.nbits = (int)n;
+      A.ncap = m_AdjWords;
       A.time = m_AdjTapeTime[k];
       A.accumulate = first ? 0 : 1;
       FX_TRY(AdjLaunch(ADJ_ADVANCE, &A, m_AdjStream));

@@ -85,6 +85,7 @@ struct AdjArgs {
   int nbits;
   float time;
   int accumulate;  // adjForce: add to the block's partials instead of overwriting
+  int ncap;        // bitmask words a slot holds; a particle needing more recorded none
 };
 
 struct Slot {
@@ -415,7 +416,7 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) 
 // as the shortest active queue holds), so the pair arithmetic runs with
 // nearly all lanes busy and no candidate is tested twice.  Each lane visits
 // its pairs in candidate order (the order of the CPU oracle's loops).  One
-// wave per block; a particle whose candidates needed more than FX_NBR_WORDS
+// wave per block; a particle whose candidates needed more than ncap
 // words has no bits and tests its candidates in place.
 #define AQ_SLOTS 1024  // ring slots per block: 16 per lane
 
@@ -473,7 +474,7 @@ __device__ __forceinline__ void ring_walk(const FParams_t& P, const AdjArgs& A, 
     int jb = 0, je = 0;
     if (use) row_of(P, base, r, A.off, A.cnt, jb, je);
     const int nw = (je - jb + 31) >> 5;
-    const int maxw = min(wave_max_i(nw), FX_NBR_WORDS);
+    const int maxw = min(wave_max_i(nw), A.ncap);
     for (int w = 0; w < maxw; w++) {
       uint word = 0;
       if (w < nw) word = A.bits[(size_t)(W + w) * NN + i];
@@ -724,7 +725,7 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) AW_FORCE_ATTR adjForce
     F.idi = 1.0f / F.di;
     F.qi = F.di - F.rest;
   }
-  const bool ovf = act && words_of(P, base, A.off, A.cnt) > FX_NBR_WORDS;
+  const bool ovf = act && words_of(P, base, A.off, A.cnt) > A.ncap;
   if (ovf) {  // no recorded bits: test the candidates in place
     const float rd2 = P.rd2;
     for (int r = 0; r < P.gridSrch * P.gridSrch; r++) {
@@ -802,7 +803,7 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjDensity(AdjArgs A) 
     D.px = a.x; D.py = a.y; D.pz = a.z;
     D.gi = a.w;
   }
-  const bool ovf = act && words_of(P, base, A.off, A.cnt) > FX_NBR_WORDS;
+  const bool ovf = act && words_of(P, base, A.off, A.cnt) > A.ncap;
   if (ovf) {
     for (int r = 0; r < P.gridSrch * P.gridSrch; r++) {
       int jb, je;

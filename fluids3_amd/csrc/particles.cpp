@@ -19,7 +19,7 @@ static const char* kKernelNames[FUNC_MAX] = {
     "fxFillUint",        "advanceParticlesClock", "fxClockTick",     "fxCountPairs",
     "insertParticlesBatch", "countingSortIndexBatch", "countingSortFullBatch", "computePressureBatch",
     "computeForceBatch", "advanceParticlesBatch", "fxCountPairsBatch", "sampleParticles",
-    "emitParticles"};
+    "emitParticles",     "fxMaxWords",       "fxMaxWordsBatch"};
 
 Particles::Particles() { memset(&m_Params, 0, sizeof(m_Params)); }
 
@@ -66,6 +66,8 @@ int Particles::Initialize(const char* path) {
   if (len != sizeof(FParams_t))
     return fx::fail(FX_EMISMATCH, "Initialize: FParams has %zu bytes, FParams_t has %zu", len,
                     sizeof(FParams_t));
+  FX_HIP(hipModuleGetGlobal(reinterpret_cast<hipDeviceptr_t*>(&m_NbrWordsSym), &len, m_Module, "FNbrWords"),
+         "Initialize");
   FX_HIP(hipStreamCreate(&m_Stream), "Initialize");
   FX_HIP(hipMalloc(&m_Tables, 2 * sizeof(cuDataX)), "Initialize");
   FX_HIP(hipMalloc((void**)&m_Clock, sizeof(float)), "Initialize");
@@ -758,6 +760,30 @@ int Particles::EmitParticles(float frame, int emit) {
   int n = mNumPoints;
   void* args[] = {&frame, &emit, &n};
   return Launch(FUNC_EMIT, (unsigned)((emit + 255) / 256), 256, args);
+}
+
+int Particles::MaxNbrWords(int* words) {
+  if (mNumPoints == 0 || m_Stats == nullptr) return fx::fail(FX_ESTATE, "max_words: call Restart first");
+  if (m_Done < FX_STAGE_PREFIX) return fx::fail(FX_ESTATE, "max_words: the counts and offsets are made by insert and scan");
+  int four = 4;
+  unsigned zero = 0;
+  void* st = m_Stats;
+  void* fargs[] = {&st, &zero, &four};
+  FX_TRY(Launch(FUNC_FILL, 1, 64, fargs));
+  if (m_Batch) {
+    unsigned nb;
+    FxBatchArgs B = BatchFor(FX_STAGE_PRESSURE, &nb);
+    void* bargs[] = {&B, &st};
+    FX_TRY(Launch(FUNC_MAXWORDS_B, nb, (unsigned)m_StageThreads[FX_STAGE_PRESSURE], bargs));
+  } else {
+    void* args[] = {&st, &mNumPoints};
+    FX_TRY(LaunchP(FUNC_MAXWORDS, FX_STAGE_PRESSURE, args));
+  }
+  unsigned h = 0;
+  FX_HIP(hipStreamSynchronize(m_Stream), "max_words");
+  FX_HIP(hipMemcpyDtoH(&h, m_Stats, sizeof(h)), "max_words");
+  *words = (int)h;
+  return FX_OK;
 }
 
 int Particles::CountPairs(int64_t* cand, int64_t* pairs) {

@@ -39,7 +39,9 @@
 #define FUNC_COUNT_PAIRS_B 18
 #define FUNC_SAMPLE 19
 #define FUNC_EMIT 20
-#define FUNC_MAX 21
+#define FUNC_MAXWORDS 21
+#define FUNC_MAXWORDS_B 22
+#define FUNC_MAX 23
 
 #include <map>
 #include <string>
@@ -107,6 +109,7 @@ class Particles {
   int TimeSteps(int nsteps, bool use_graph, double* ms_total);
   int BenchSteps(int nsteps, bool use_graph, double* ms_total, double* stage_ms);
   int CountPairs(int64_t* cand, int64_t* pairs);
+  int MaxNbrWords(int* words);  // after insert and scan: the largest stencil's bitmask words
   int Autotune(int nsteps, int* chosen, double* stage_ms);  // autotune.cpp
   int SavePoints(const char* path);                         // points_io.cpp
   int SampleParticles(float* brick, const int* res, const float* bmin, const float* bmax, float scalar);
@@ -167,6 +170,7 @@ class Particles {
   int m_Done = -1;
   float* m_Clock = nullptr;
   void* m_Stats = nullptr;  // two uint64 counters of fxCountPairs
+  void* m_NbrWordsSym = nullptr;  // the module's FNbrWords (bitmask words a particle may record)
 
   hipGraph_t m_Graph = nullptr;
   hipGraphExec_t m_GraphExec = nullptr;
@@ -202,7 +206,9 @@ class Particles {
   std::vector<float> m_AdjTapeTime;      // time of each recorded step of the segment
   char* m_AdjCkpt = nullptr;             // (segs+1) x 44 B x N
   char* m_AdjTape = nullptr;             // every x (60 B x N + 8 B x cells)
-  char* m_AdjBits = nullptr;             // every x FX_NBR_WORDS x 4 B x N: each recorded step's bitmask
+  char* m_AdjBits = nullptr;             // every x m_AdjWords x 4 B x N: each recorded step's bitmask
+  int m_AdjWords = FX_NBR_WORDS;         // bitmask words a tape slot holds per particle
+  int* m_AdjWordCaps = nullptr;          // device {FX_NBR_WORDS, m_AdjWords}: values copied into FNbrWords
   void* m_AdjArec = nullptr;             // 48 B x N pair records of the force adjoint
   void* m_AdjDrec = nullptr;             // 16 B x N pair records of the density adjoint
   void* m_AdjTabs = nullptr;             // FAccel tables routing ANBRBITS into the tape

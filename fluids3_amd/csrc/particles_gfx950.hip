@@ -31,6 +31,13 @@ __constant__ cuDataX FPntTmp;
 __constant__ cuDataX FAccel;
 
 #define SCAN_BLOCKSIZE 512  // threads per scan block; each covers 1024 cells
+
+// Words of neighbour bitmask a particle may record: FX_NBR_WORDS unless the
+// host lowers it while the bitmask buffer is a smaller one (a reverse-mode
+// tape slot).  A particle whose stencil rows need more records no bits, and
+// the force stage tests its candidates in place.
+__constant__ int FNbrWords = FX_NBR_WORDS;
+__device__ __forceinline__ int nbr_cap() { return min(FNbrWords, FX_NBR_WORDS); }
 #define FX_RANK_CAP 65536   // cells with more members keep atomic order
 
 // ---------------------------------------------------------------- helpers
@@ -486,7 +493,7 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
   const int srch = P.gridSrch;
   const float rd2 = P.rd2, d2 = P.d2;
   const size_t NN = (size_t)N;
-  const bool rec = wbits && act && nbr_words(P, gc, off, cnt) <= FX_NBR_WORDS;
+  const bool rec = wbits && act && nbr_words(P, gc, off, cnt) <= nbr_cap();
   float sum = 0.0f;
   int W = 0;  // first word of the current row
   for (int r = 0; r < srch * srch; r++) {
@@ -757,7 +764,7 @@ __device__ __forceinline__ void force_impl(const FParams_t& P, const Span S, int
     I.pi = (I.di - P.prest_dens) * P.pintstiff;
   }
   // a particle without recorded bits (too many candidates) tests its own
-  const bool ovf = act && nbr_words(P, gc, off, cnt) > FX_NBR_WORDS;
+  const bool ovf = act && nbr_words(P, gc, off, cnt) > nbr_cap();
   if (ovf) {
     for (int r = 0; r < srch * srch; r++) {
       int jb, je;
@@ -1011,7 +1018,7 @@ __device__ __forceinline__ void force_q_impl(const FParams_t& P, const Span S, i
 #pragma unroll
     for (int r = 0; r < R; r++) E_[r] = G[r] = 0;
   }
-  const bool ovf = act && nwords > FX_NBR_WORDS;
+  const bool ovf = act && nwords > nbr_cap();
   if (ovf) {  // no recorded bits: test every candidate
     const int base = stencil_base(P, gc);
     for (int r = 0; r < R; r++) {
@@ -1061,6 +1068,28 @@ extern "C" __global__ __launch_bounds__(1024) FW_ATTR void computeForceBatch(Bat
 }
 
 // ------------------------------------------------------ neighbour statistics
+
+// The largest number of bitmask words any in-range particle's stencil needs,
+// on a state whose counts and offsets are current (after insert and scan; the
+// particles need not be sorted): out[0] = max.  Sizes the reverse mode's tape.
+__device__ __forceinline__ void max_words_impl(const FParams_t& P, const Span S, uint* out) {
+  const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
+  if (i >= S.p1) return;
+  const int gc = pbuf<const int>(FPnts, FGCELL)[i];
+  if (gc == FX_GRID_UNDEF) return;
+  const int w = nbr_words(P, gc, pbuf<const uint>(FAccel, AGRIDOFF), pbuf<const uint>(FAccel, AGRIDCNT));
+  atomicMax(out, (uint)w);
+}
+
+extern "C" __global__ void fxMaxWords(uint* out, int pnum) {
+  max_words_impl(*cparams(&FParams), Span{0, pnum, 0, (int)blockIdx.x}, out);
+}
+
+extern "C" __global__ void fxMaxWordsBatch(BatchArgs B, uint* out) {
+  const FParams_t* P;
+  const Span S = batch_span(B, blockIdx.x, P);
+  max_words_impl(*P, S, out);
+}
 
 // Sums over all in-range particles of the stencil candidates (members of the
 // stencil cells, self included) and of the in-radius pairs (0 < d2 < rd2),

@@ -161,6 +161,34 @@ def test_adjoint_one_tape_buffer(ps, monkeypatch):
             assert g[k][0] == ref[k][0], (every, k)
 
 
+def test_adjoint_tape_bitmask_size(ps, monkeypatch):
+    """The tape's bitmask holds the words the densest stencil needs (twice
+    that, at least 32); with only 4 words a slot most particles record no
+    bits and both the forward force and the adjoint test their candidates in
+    place: same bits, over segments recomputed from checkpoints."""
+    sc = scenes.scaled_dam_break(5000)
+    T = 9
+    target = target_positions(ps, sc, T)
+    ref = gpu_grad(ps, sc, T, 4, target)
+    full = ps.AdjointMemory()
+    monkeypatch.setenv("FX_ADJOINT_TAPE_WORDS", "4")
+    g = gpu_grad(ps, sc, T, 4, target)
+    assert ps.AdjointMemory() < full
+    for k in ("gpos", "gvel", "gveval"):
+        assert bits_equal(g[k], ref[k]), k
+    for k in ("loss", "gvisc", "gk"):
+        assert g[k][0] == ref[k][0], k
+    # the system goes on from the rollout's end state as a plain run does
+    ps.Run(2)
+    after = ps.state()
+    with Particles() as other:
+        other.Restart(sc)
+        other.Run(T + 2)
+        want = other.state()
+    for k in ("pos", "vel", "veval", "id"):
+        assert bits_equal(after[k], want[k]), k
+
+
 def test_adjoint_leaves_the_rollout_end_state(ps):
     sc = scenes.scaled_dam_break(4000)
     T = 7
