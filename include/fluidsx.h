@@ -233,11 +233,16 @@ int fx_autotune(fx_system* sys, int32_t nsteps, int32_t* threads_out, double* ms
 
 /* Loads the adjoint code object (NULL: adjoint_gfx950.hsaco beside the step's
  * code object) and allocates checkpoints and tape for the current Restart.
- * checkpoint_every <= 0 chooses the fewest segments, of equal length, whose
- * tape and checkpoints fit in 60 % of the device memory free at the call (the
- * backward recomputes all segments but the last; a rollout in one segment
- * needs no recomputation).  When a second tape buffer also fits, the backward
- * recomputes each segment while the adjoint of the next one runs. */
+ * It runs the insert and prefix stages on the current state (reordering
+ * nothing) to size each tape slot's neighbour bitmask: twice the words the
+ * densest stencil needs, 32 to FX_NBR_WORDS per particle; a particle needing
+ * more during the rollout records no bits and is walked by testing its
+ * candidates, with the same results.  checkpoint_every <= 0 chooses the
+ * fewest segments, of equal length, whose tape and checkpoints fit in 60 % of
+ * the device memory free at the call (the backward recomputes all segments
+ * but the last; a rollout in one segment needs no recomputation).  When a
+ * second tape buffer also fits, the backward recomputes each segment while
+ * the adjoint of the next one runs. */
 int fx_adjoint_configure(fx_system* sys, const char* code_object_path, int32_t nsteps,
                          int32_t checkpoint_every);
 /* The segment length in use (the chosen one when configured with <= 0). */
