@@ -386,7 +386,7 @@ extern "C" __global__ void countingSortIndex(int pnum) {
 }
 
 extern "C" __global__ void countingSortFull(int pnum) {
-  sort_full_impl(*cparams(&FParams), Span{0, pnum, 0, (int)blockIdx.x});
+  sort_full_impl(*cparams(&FParams), Span{0, pnum, 0, xcd_block()});
 }
 
 extern "C" __global__ void countingSortIndexBatch(BatchArgs B) {
@@ -397,7 +397,7 @@ extern "C" __global__ void countingSortIndexBatch(BatchArgs B) {
 
 extern "C" __global__ void countingSortFullBatch(BatchArgs B) {
   const FParams_t* P;
-  const Span S = batch_span(B, blockIdx.x, P);
+  const Span S = batch_span(B, xcd_block(), P);
   sort_full_impl(*P, S);
 }
 
