@@ -336,6 +336,109 @@ def test_advance_wall_bands(axis, side):
     assert np.allclose(a, want, rtol=1e-4, atol=1e-3)
 
 
+def wall_scene(slope=0.0, friction=0.0, wall_force=0.0, wall_freq=0.0):
+    """The 40^3 box with the floor slope, friction and moving X-min wall set,
+    and an acceleration limit far above the wall penalties, so that each
+    wall's term shows unscaled."""
+    sc = small_box_scene()
+    sc.bound_slope, sc.bound_friction = slope, friction
+    sc.bound_wall_force, sc.bound_wall_freq = wall_force, wall_freq
+    sc.accel_limit = 1.0e5
+    return sc
+
+
+def close3(a, want, name):
+    """Float32 integrator against a closed form in double: the rounding of
+    a*dt + v in float32 bounds the difference by ~1e-5 of |a|."""
+    a, want = np.asarray(a, np.float64), np.asarray(want, np.float64)
+    tol = 2e-5 * max(1.0, float(np.abs(want).max()))
+    assert np.abs(a - want).max() <= tol, f"{name}: {a.tolist()} vs closed form {want.tolist()}"
+
+
+def test_advance_sloped_floor_with_friction():
+    """Y-min wall on the sloped floor y = bmin.y + (x - bmin.x) slope
+    (particles.cu:258-266): inward normal (-slope, 1 - slope, 0), penalty
+    stiff*diff - damp*(n.veval) along it, and friction -veval*friction on all
+    three axes."""
+    s, fr = 0.3, 2.0
+    P = params_for(wall_scene(slope=s, friction=fr))
+    bmin = np.array(P.bound_min.tolist(), np.float64)
+    x, z = 0.0, 0.0
+    floor = bmin[1] + (x - bmin[0]) * s  # -14: the slope raises the floor by 6 at x = 0
+    pos = (x, floor + 1.0, z)            # 1 world unit above it (0.004 m < pradius)
+    vev = np.array([0.2, -0.3, 0.1])
+    a, _ = accel_after(P, pos, (0, 0, 0), vev, (0, 0, 0))
+    n = np.array([-s, 1.0 - s, 0.0])
+    diff = P.pradius - 1.0 * P.sim_scale
+    adj = P.bound_stiff * diff - P.bound_damp * float(n @ vev)
+    want = n * adj - vev * fr + np.array(P.gravity.tolist())
+    close3(a, want, "sloped floor")
+    # a slope-blind floor would leave this particle (6 units above y = bmin.y) untouched
+    assert abs(a[1] - P.gravity.y) > 10.0
+
+
+def test_advance_sloped_floor_band_edge():
+    """Just above the band (diff < EPSILON) the sloped floor applies nothing."""
+    s = 0.3
+    P = params_for(wall_scene(slope=s, friction=2.0))
+    bmin = np.array(P.bound_min.tolist(), np.float64)
+    floor = bmin[1] + (0.0 - bmin[0]) * s
+    above = P.pradius / P.sim_scale + 0.5  # diff = pradius - 3 ss < 0
+    a, _ = accel_after(P, (0.0, floor + above, 0.0), (0, 0, 0), (0.2, -0.3, 0.1), (0, 0, 0))
+    close3(a, P.gravity.tolist(), "above the sloped floor")
+
+
+def test_advance_ceiling_with_friction():
+    """Y-max wall (particles.cu:267-272): normal (0, -1, 0) and the same
+    friction term as the floor."""
+    fr = 2.0
+    P = params_for(wall_scene(slope=0.3, friction=fr))
+    bmax = np.array(P.bound_max.tolist(), np.float64)
+    vev = np.array([0.2, 0.3, -0.1])
+    a, _ = accel_after(P, (0.0, bmax[1] - 1.0, 0.0), (0, 0, 0), vev, (0, 0, 0))
+    n = np.array([0.0, -1.0, 0.0])
+    adj = P.bound_stiff * (P.pradius - 1.0 * P.sim_scale) - P.bound_damp * float(n @ vev)
+    want = n * adj - vev * fr + np.array(P.gravity.tolist())
+    close3(a, want, "ceiling")
+
+
+def test_advance_friction_only_on_y_walls():
+    """The X and Z walls carry no friction term (particles.cu:273-300)."""
+    fr = 2.0
+    P = params_for(wall_scene(friction=fr))
+    bmax = np.array(P.bound_max.tolist(), np.float64)
+    vev = np.array([-0.2, 0.3, 0.1])
+    a, _ = accel_after(P, (bmax[0] - 1.0, 0.0, 0.0), (0, 0, 0), vev, (0, 0, 0))
+    n = np.array([-1.0, 0.0, 0.0])
+    adj = P.bound_stiff * (P.pradius - 1.0 * P.sim_scale) - P.bound_damp * float(n @ vev)
+    close3(a, n * adj + np.array(P.gravity.tolist()), "X-max wall")
+
+
+@pytest.mark.parametrize("t,z", [(0.25, 10.0), (1.3, -15.0)])
+def test_advance_moving_x_wall(t, z):
+    """X-min wall (particles.cu:273-281): the wall sits at bmin.x + wall with
+    wall = (sin(t freq + z/400) 0.5 + 0.5) wall_force, and its stiffness term
+    is wall*stiff*diff (damping unscaled).  Two (t, z) points with different
+    amplitudes."""
+    force, freq = 2.0, 3.0
+    P = params_for(wall_scene(wall_force=force, wall_freq=freq))
+    bmin = np.array(P.bound_min.tolist(), np.float64)
+    wall = (math.sin(t * freq + z / 400.0) * 0.5 + 0.5) * force
+    gap = 0.5
+    vev = np.array([-0.25, 0.1, 0.05])
+    st = oracle.make_state(P, np.array([[bmin[0] + wall + gap, 0.0, z]], F32), veval=np.array([vev], F32), time=t)
+    st = oracle.run(P, st, 1, oracle.STAGE_INSERT)
+    st["force"] = np.zeros((1, 3), F32)
+    st = oracle.run(P, st, 1, oracle.STAGE_ADVANCE)
+    a = st["vel"][0].astype(np.float64) / P.dt
+    diff = P.pradius - gap * P.sim_scale
+    adj = wall * P.bound_stiff * diff - P.bound_damp * float(vev[0])
+    want = np.array([adj, 0.0, 0.0]) + np.array(P.gravity.tolist())
+    close3(a, want, f"moving wall at t={t}, z={z}")
+    # a fixed wall would give stiff*diff at this amplitude: the scaling is visible
+    assert abs(wall - 1.0) > 0.3
+
+
 def test_advance_accel_and_velocity_limits():
     P = params_for(small_box_scene())
     a, _ = accel_after(P, (0, 0, 0), (0, 0, 0), (0, 0, 0), (1e7, 0, 0))
