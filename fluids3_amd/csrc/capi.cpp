@@ -163,6 +163,26 @@ int fx_scan(fx_system* s, const uint32_t* in, uint32_t* out, int32_t n) {
   return s->p.Scan(in, out, n);
 }
 
+int fx_set_stage_ppt(fx_system* s, int32_t stage, int32_t ppt) {
+  FX_CHECK_SYS(s);
+  return s->p.SetStagePpt(stage, ppt);
+}
+
+int32_t fx_stage_ppt(const fx_system* s, int32_t stage) {
+  if (s == nullptr || stage < 0 || stage >= FX_NUM_STAGES) return 0;
+  return s->p.StagePpt(stage);
+}
+
+int fx_set_persistent(fx_system* s, int32_t mode) {
+  FX_CHECK_SYS(s);
+  return s->p.SetPersistent(mode);
+}
+
+int fx_mark(fx_system* s, int32_t tag) {
+  FX_CHECK_SYS(s);
+  return s->p.Mark(tag);
+}
+
 int fx_graph_capture(fx_system* s) {
   FX_CHECK_SYS(s);
   return s->p.CaptureGraph();

@@ -19,7 +19,7 @@ static const char* kKernelNames[FUNC_MAX] = {
     "fxFillUint",        "advanceParticlesClock", "fxClockTick",     "fxCountPairs",
     "insertParticlesBatch", "countingSortIndexBatch", "countingSortFullBatch", "computePressureBatch",
     "computeForceBatch", "advanceParticlesBatch", "fxCountPairsBatch", "sampleParticles",
-    "emitParticles",     "fxMaxWords",       "fxMaxWordsBatch"};
+    "emitParticles",     "fxMaxWords",       "fxMaxWordsBatch",  "fxMark"};
 
 Particles::Particles() { memset(&m_Params, 0, sizeof(m_Params)); }
 
@@ -48,6 +48,7 @@ Particles::~Particles() {
   if (m_Tables) (void)hipFree(m_Tables);
   if (m_Clock) (void)hipFree(m_Clock);
   if (m_Stats) (void)hipFree(m_Stats);
+  if (m_WorkQ) (void)hipFree(m_WorkQ);
   if (m_Stream) (void)hipStreamDestroy(m_Stream);
   if (m_Module) (void)hipModuleUnload(m_Module);
 }
@@ -68,10 +69,30 @@ int Particles::Initialize(const char* path) {
                     sizeof(FParams_t));
   FX_HIP(hipModuleGetGlobal(reinterpret_cast<hipDeviceptr_t*>(&m_NbrWordsSym), &len, m_Module, "FNbrWords"),
          "Initialize");
+  // this host allocates the tail slot and runs the index sort (a host that
+  // binds the module as the reference does leaves FHostCaps at 0)
+  hipDeviceptr_t caps = nullptr;
+  FX_HIP(hipModuleGetGlobal(&caps, &len, m_Module, "FHostCaps"), "Initialize");
+  if (len != sizeof(int)) return fx::fail(FX_EMISMATCH, "Initialize: FHostCaps has %zu bytes", len);
+  int capv = FX_CAP_TAIL | FX_CAP_RANKED;
+  FX_HIP(hipMemcpyHtoD(caps, &capv, sizeof(int)), "Initialize");
   FX_HIP(hipStreamCreate(&m_Stream), "Initialize");
   FX_HIP(hipMalloc(&m_Tables, 2 * sizeof(cuDataX)), "Initialize");
   FX_HIP(hipMalloc((void**)&m_Clock, sizeof(float)), "Initialize");
   FX_HIP(hipMalloc(&m_Stats, 2 * sizeof(unsigned long long)), "Initialize");
+  FX_HIP(hipMalloc((void**)&m_WorkQ, 8 * FX_NUM_STAGES * sizeof(uint32_t)), "Initialize");
+  FX_HIP(hipModuleGetGlobal(reinterpret_cast<hipDeviceptr_t*>(&m_PptSym), &len, m_Module, "FPpt"), "Initialize");
+  if (len != 8 * sizeof(int)) return fx::fail(FX_EMISMATCH, "Initialize: FPpt has %zu bytes", len);
+  {
+    int dev = 0;
+    FX_HIP(hipGetDevice(&dev), "Initialize");
+    FX_HIP(hipDeviceGetAttribute(&m_NumCU, hipDeviceAttributeMultiprocessorCount, dev), "Initialize");
+  }
+  FX_HIP(hipModuleGetGlobal(reinterpret_cast<hipDeviceptr_t*>(&m_PersistSym), &len, m_Module, "FPersistQ"),
+         "Initialize");
+  if (len != 8 * sizeof(uint64_t)) return fx::fail(FX_EMISMATCH, "Initialize: FPersistQ has %zu bytes", len);
+  FX_TRY(UploadPpt());
+  FX_TRY(UploadPersistQ());
   m_Params.numThreads = 256;
   return FX_OK;
 }
@@ -87,7 +108,83 @@ int Particles::LaunchP(int fid, int stage, void** args) {
   const int waves = (mNumPoints + 63) / 64 * 64;
   const int threads = m_StageThreads[stage] < waves ? m_StageThreads[stage] : waves;
   if (threads <= 0) return FX_OK;
-  return Launch(fid, (unsigned)((mNumPoints + threads - 1) / threads), (unsigned)threads, args);
+  const int per_block = threads * m_StagePpt[stage];
+  return Launch(fid, (unsigned)((mNumPoints + per_block - 1) / per_block), (unsigned)threads, args);
+}
+
+int Particles::UploadPpt() {
+  int v[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+  for (int k = 0; k < FX_NUM_STAGES; k++) v[k] = m_StagePpt[k];
+  FX_HIP(hipStreamSynchronize(m_Stream ? m_Stream : nullptr), "ppt");
+  FX_HIP(hipMemcpyHtoD(reinterpret_cast<hipDeviceptr_t>(m_PptSym), v, sizeof(v)), "ppt");
+  return FX_OK;
+}
+
+int Particles::SetStagePpt(int stage, int ppt) {
+  if (stage < 0 || stage >= FX_NUM_STAGES) return fx::fail(FX_EINVAL, "SetStagePpt: bad stage %d", stage);
+  if (ppt < 1 || ppt > 8) return fx::fail(FX_EINVAL, "SetStagePpt: %d particles per thread is not in [1, 8]", ppt);
+  if (ppt != 1 && stage != FX_STAGE_INSERT && stage != FX_STAGE_SORT && stage != FX_STAGE_ADVANCE)
+    return fx::fail(FX_EINVAL, "SetStagePpt: stage %d maps one particle to one lane (the pair stages batch work "
+                    "per block and per wave instead)", stage);
+  if (m_StagePpt[stage] == ppt) return FX_OK;
+  ReleaseGraph();
+  m_StagePpt[stage] = ppt;
+  FX_TRY(UploadPpt());
+  return BuildBlockTables();
+}
+
+int Particles::SetPersistent(int mode) {
+  if (mode & ~(FX_PERSIST_BATCH | FX_PERSIST_SCENE)) return fx::fail(FX_EINVAL, "SetPersistent: mode %d", mode);
+  ReleaseGraph();
+  m_Persistent = (mode & FX_PERSIST_BATCH) != 0;
+  m_ScenePersistent = (mode & FX_PERSIST_SCENE) != 0;
+  return UploadPersistQ();
+}
+
+int Particles::UploadPersistQ() {
+  // the single-scene density and force kernels find their claim counters here
+  uint64_t q[8] = {};
+  if (m_ScenePersistent) {
+    q[FX_STAGE_PRESSURE] = (uint64_t)(uintptr_t)(m_WorkQ + 8 * FX_STAGE_PRESSURE);
+    q[FX_STAGE_FORCE] = (uint64_t)(uintptr_t)(m_WorkQ + 8 * FX_STAGE_FORCE);
+  }
+  FX_HIP(hipStreamSynchronize(m_Stream ? m_Stream : nullptr), "persistent");
+  FX_HIP(hipMemcpyHtoD(reinterpret_cast<hipDeviceptr_t>(m_PersistSym), q, sizeof(q)), "persistent");
+  return FX_OK;
+}
+
+int Particles::LaunchPair(int fid, int stage) {
+  // the single-scene density / force launch: one block per block of
+  // particles, or (persistent) as many as the device holds, after clearing
+  // the stage's claim counters
+  void* args[] = {&mNumPoints};
+  if (!m_ScenePersistent) return LaunchP(fid, stage, args);
+  const int waves = (mNumPoints + 63) / 64 * 64;
+  const int threads = m_StageThreads[stage] < waves ? m_StageThreads[stage] : waves;
+  if (threads <= 0) return FX_OK;
+  unsigned grid = (unsigned)((mNumPoints + threads - 1) / threads);
+  int resident = 0;
+  FX_TRY(ResidentBlocks(fid, threads, &resident));
+  if ((unsigned)resident < grid) grid = (unsigned)resident;
+  int eight = 8;
+  unsigned zero = 0;
+  void* q = m_WorkQ + 8 * stage;
+  void* fargs[] = {&q, &zero, &eight};
+  FX_TRY(Launch(FUNC_FILL, 1, 64, fargs));
+  return Launch(fid, grid, (unsigned)threads, args);
+}
+
+int Particles::ResidentBlocks(int fid, int threads, int* blocks) {
+  const auto key = std::make_pair(fid, threads);
+  auto it = m_Resident.find(key);
+  if (it == m_Resident.end()) {
+    int per_cu = 0;
+    FX_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, m_Func[fid], threads, 0), "occupancy");
+    if (per_cu < 1) per_cu = 1;
+    it = m_Resident.emplace(key, per_cu * (m_NumCU > 0 ? m_NumCU : 1)).first;
+  }
+  *blocks = it->second;
+  return FX_OK;
 }
 
 int Particles::Restart(const fx_scene& sc) {
@@ -177,7 +274,7 @@ int Particles::BuildBlockTables() {
   m_BTab.clear();
   if (!m_Batch) return FX_OK;
   for (int k = 0; k < FX_NUM_STAGES; k++) {
-    const int T = m_StageThreads[k];
+    const int T = m_StageThreads[k] * m_StagePpt[k];  // particles per block
     if (m_BTab.count(T)) continue;
     std::vector<int> tab;
     for (int s = 0; s + 1 < (int)m_PBase.size(); s++) {
@@ -195,10 +292,29 @@ int Particles::BuildBlockTables() {
   return FX_OK;
 }
 
-FxBatchArgs Particles::BatchFor(int stage, unsigned* blocks) const {
-  const auto it = m_BTab.find(m_StageThreads[stage]);
-  *blocks = it == m_BTab.end() ? 0u : (unsigned)it->second.second;
-  return FxBatchArgs{m_DParams, m_DPBase, m_DCBase, it == m_BTab.end() ? nullptr : it->second.first};
+int Particles::BatchLaunch(int fid, int stage, bool persistent, void* extra0, void* extra1, void* extra2) {
+  // One launch over every scene of the batch.  Non-persistent: one block per
+  // block-table entry.  Persistent: as many blocks as the device holds at
+  // once (capped by the entries), pulling entries from the stage's claim
+  // counters, which a fill kernel clears first.
+  const int T = m_StageThreads[stage];
+  const auto it = m_BTab.find(T * m_StagePpt[stage]);
+  if (it == m_BTab.end()) return fx::fail(FX_ESTATE, "batch launch: no block table for stage %d", stage);
+  FxBatchArgs B{m_DParams, m_DPBase, m_DCBase, it->second.first, nullptr, it->second.second, 0};
+  unsigned grid = (unsigned)it->second.second;
+  if (persistent && m_Persistent) {
+    B.queue = m_WorkQ + 8 * stage;
+    int resident = 0;
+    FX_TRY(ResidentBlocks(fid, T, &resident));
+    if ((unsigned)resident < grid) grid = (unsigned)resident;
+    int eight = 8;
+    unsigned zero = 0;
+    void* q = B.queue;
+    void* fargs[] = {&q, &zero, &eight};
+    FX_TRY(Launch(FUNC_FILL, 1, 64, fargs));
+  }
+  void* args[4] = {&B, extra0, extra1, extra2};
+  return Launch(fid, grid, (unsigned)T, args);
 }
 
 int Particles::SceneRange(int s, int* p0, int* np, int* c0, int* nc) const {
@@ -370,10 +486,7 @@ int Particles::InsertParticles() {
   void* fargs[] = {&cnt, &zero, &ncell};
   FX_TRY(Launch(FUNC_FILL, (unsigned)((ncell + 255) / 256), 256, fargs));
   if (m_Batch) {
-    unsigned nb;
-    FxBatchArgs B = BatchFor(FX_STAGE_INSERT, &nb);
-    void* bargs[] = {&B};
-    FX_TRY(Launch(FUNC_INSERT_B, nb, (unsigned)m_StageThreads[FX_STAGE_INSERT], bargs));
+    FX_TRY(BatchLaunch(FUNC_INSERT_B, FX_STAGE_INSERT, true, nullptr, nullptr, nullptr));
   } else {
     void* args[] = {&mNumPoints};
     FX_TRY(LaunchP(FUNC_INSERT, FX_STAGE_INSERT, args));
@@ -434,11 +547,8 @@ int Particles::CountingSort() {
   m_Parity ^= 1;
   FX_TRY(PublishTables());
   if (m_Batch) {
-    unsigned nb;
-    FxBatchArgs B = BatchFor(FX_STAGE_SORT, &nb);
-    void* bargs[] = {&B};
-    FX_TRY(Launch(FUNC_SORT_INDEX_B, nb, (unsigned)m_StageThreads[FX_STAGE_SORT], bargs));
-    FX_TRY(Launch(FUNC_SORT_FULL_B, nb, (unsigned)m_StageThreads[FX_STAGE_SORT], bargs));
+    FX_TRY(BatchLaunch(FUNC_SORT_INDEX_B, FX_STAGE_SORT, true, nullptr, nullptr, nullptr));
+    FX_TRY(BatchLaunch(FUNC_SORT_FULL_B, FX_STAGE_SORT, true, nullptr, nullptr, nullptr));
   } else {
     void* args[] = {&mNumPoints};
     FX_TRY(LaunchP(FUNC_SORT_INDEX, FX_STAGE_SORT, args));
@@ -450,13 +560,9 @@ int Particles::CountingSort() {
 
 int Particles::ComputePressure() {
   if (m_Batch) {
-    unsigned nb;
-    FxBatchArgs B = BatchFor(FX_STAGE_PRESSURE, &nb);
-    void* bargs[] = {&B, &mNumPoints};
-    FX_TRY(Launch(FUNC_PRESS_B, nb, (unsigned)m_StageThreads[FX_STAGE_PRESSURE], bargs));
+    FX_TRY(BatchLaunch(FUNC_PRESS_B, FX_STAGE_PRESSURE, true, &mNumPoints, nullptr, nullptr));
   } else {
-    void* args[] = {&mNumPoints};
-    FX_TRY(LaunchP(FUNC_COMPUTE_PRESS, FX_STAGE_PRESSURE, args));
+    FX_TRY(LaunchPair(FUNC_COMPUTE_PRESS, FX_STAGE_PRESSURE));
   }
   m_Done = FX_STAGE_PRESSURE;
   return FX_OK;
@@ -464,13 +570,9 @@ int Particles::ComputePressure() {
 
 int Particles::ComputeForce() {
   if (m_Batch) {
-    unsigned nb;
-    FxBatchArgs B = BatchFor(FX_STAGE_FORCE, &nb);
-    void* bargs[] = {&B, &mNumPoints};
-    FX_TRY(Launch(FUNC_FORCE_B, nb, (unsigned)m_StageThreads[FX_STAGE_FORCE], bargs));
+    FX_TRY(BatchLaunch(FUNC_FORCE_B, FX_STAGE_FORCE, true, &mNumPoints, nullptr, nullptr));
   } else {
-    void* args[] = {&mNumPoints};
-    FX_TRY(LaunchP(FUNC_COMPUTE_FORCE, FX_STAGE_FORCE, args));
+    FX_TRY(LaunchPair(FUNC_COMPUTE_FORCE, FX_STAGE_FORCE));
   }
   m_Done = FX_STAGE_FORCE;
   return FX_OK;
@@ -481,12 +583,9 @@ int Particles::Advance() {
   float dt = m_Params.dt, ss = m_Params.sim_scale;
   int n = m_Params.pnum;
   if (m_Batch) {
-    unsigned nb;
-    FxBatchArgs B = BatchFor(FX_STAGE_ADVANCE, &nb);
     float time = m_Params.time;
     void* clk = m_Capturing ? (void*)m_Clock : nullptr;
-    void* bargs[] = {&B, &clk, &time};
-    FX_TRY(Launch(FUNC_ADVANCE_B, nb, (unsigned)m_StageThreads[FX_STAGE_ADVANCE], bargs));
+    FX_TRY(BatchLaunch(FUNC_ADVANCE_B, FX_STAGE_ADVANCE, true, &clk, &time, nullptr));
     if (!m_Capturing) return FX_OK;
     void* targs[] = {&m_Clock, &dt};
     return Launch(FUNC_CLOCK_TICK, 1, 64, targs);
@@ -682,6 +781,12 @@ int Particles::Scan(const uint32_t* in, uint32_t* out, int n) {
   return st;
 }
 
+int Particles::Mark(int tag) {
+  if (m_Module == nullptr) return fx::fail(FX_ESTATE, "mark: call Initialize first");
+  void* args[] = {&tag};
+  return Launch(FUNC_MARK, 1, 64, args);
+}
+
 int Particles::Synchronize() {
   FX_HIP(hipStreamSynchronize(m_Stream), "synchronize");
   return FX_OK;
@@ -771,10 +876,7 @@ int Particles::MaxNbrWords(int* words) {
   void* fargs[] = {&st, &zero, &four};
   FX_TRY(Launch(FUNC_FILL, 1, 64, fargs));
   if (m_Batch) {
-    unsigned nb;
-    FxBatchArgs B = BatchFor(FX_STAGE_PRESSURE, &nb);
-    void* bargs[] = {&B, &st};
-    FX_TRY(Launch(FUNC_MAXWORDS_B, nb, (unsigned)m_StageThreads[FX_STAGE_PRESSURE], bargs));
+    FX_TRY(BatchLaunch(FUNC_MAXWORDS_B, FX_STAGE_PRESSURE, false, &st, nullptr, nullptr));
   } else {
     void* args[] = {&st, &mNumPoints};
     FX_TRY(LaunchP(FUNC_MAXWORDS, FX_STAGE_PRESSURE, args));
@@ -795,10 +897,7 @@ int Particles::CountPairs(int64_t* cand, int64_t* pairs) {
   void* fargs[] = {&st, &zero, &four};
   FX_TRY(Launch(FUNC_FILL, 1, 64, fargs));
   if (m_Batch) {
-    unsigned nb;
-    FxBatchArgs B = BatchFor(FX_STAGE_PRESSURE, &nb);
-    void* bargs[] = {&B, &st};
-    FX_TRY(Launch(FUNC_COUNT_PAIRS_B, nb, (unsigned)m_StageThreads[FX_STAGE_PRESSURE], bargs));
+    FX_TRY(BatchLaunch(FUNC_COUNT_PAIRS_B, FX_STAGE_PRESSURE, false, &st, nullptr, nullptr));
   } else {
     void* args[] = {&st, &mNumPoints};
     FX_TRY(LaunchP(FUNC_COUNT_PAIRS, FX_STAGE_PRESSURE, args));

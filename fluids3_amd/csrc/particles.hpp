@@ -41,7 +41,8 @@
 #define FUNC_EMIT 20
 #define FUNC_MAXWORDS 21
 #define FUNC_MAXWORDS_B 22
-#define FUNC_MAX 23
+#define FUNC_MARK 23
+#define FUNC_MAX 24
 
 #include <map>
 #include <string>
@@ -53,6 +54,9 @@ struct FxBatchArgs {
   const void* pbase;
   const void* cbase;
   const void* btab;
+  void* queue;  // claim counters of a persistent launch, else null
+  int nitems;   // block-table entries
+  int pad;
 };
 
 class Particles {
@@ -79,6 +83,12 @@ class Particles {
   int SetBlockSize(int threads);
   int SetStageBlockSize(int stage, int threads);
   int StageBlockSize(int stage) const { return m_StageThreads[stage]; }
+  // particles per thread of the insert, sort and advance stages (1..8)
+  int SetStagePpt(int stage, int ppt);
+  int StagePpt(int stage) const { return m_StagePpt[stage]; }
+  // persistent (fixed-grid, work-queue) launches: FX_PERSIST_* bits
+  int SetPersistent(int mode);
+  int Persistent() const { return (m_Persistent ? FX_PERSIST_BATCH : 0) | (m_ScenePersistent ? FX_PERSIST_SCENE : 0); }
 
   // the step (particles.cpp:382-400)
   int Run();
@@ -103,6 +113,7 @@ class Particles {
   int DownloadAccel(int buf, void* host, int64_t count);
   int Synchronize();
   int Scan(const uint32_t* in, uint32_t* out, int n);
+  int Mark(int tag);
 
   // timing
   int ProfileStages(int nsteps, double* ms_out);
@@ -137,7 +148,13 @@ class Particles {
   int PublishTables();
   void ReleaseGraph();
   int BuildBlockTables();
-  FxBatchArgs BatchFor(int stage, unsigned* blocks) const;
+  // kernel arguments and grid of a batch launch of function fid for a stage;
+  // persistent launches reset the stage's claim counters first
+  int BatchLaunch(int fid, int stage, bool persistent, void* extra0, void* extra1, void* extra2);
+  int ResidentBlocks(int fid, int threads, int* blocks);
+  int UploadPpt();
+  int UploadPersistQ();
+  int LaunchPair(int fid, int stage);
   int FinishRestart(int n);
   // adjoint helpers
   void AdjRelease();
@@ -180,6 +197,14 @@ class Particles {
   // sort, pressure, force, advance): the sizes fx_autotune keeps choosing on
   // the dam-break and beach scenes at 4M particles
   int m_StageThreads[FX_NUM_STAGES] = {128, 256, 512, 256, 256, 64};
+  int m_StagePpt[FX_NUM_STAGES] = {1, 1, 1, 1, 1, 1};
+  void* m_PptSym = nullptr;     // module constant FPpt
+  bool m_Persistent = true;     // batch kernels run as persistent launches
+  bool m_ScenePersistent = false;  // single-scene density and force run as persistent launches
+  void* m_PersistSym = nullptr;    // module constant FPersistQ
+  uint32_t* m_WorkQ = nullptr;  // 8 claim counters per stage
+  int m_NumCU = 0;
+  std::map<std::pair<int, int>, int> m_Resident;  // (function, threads) -> resident blocks on the device
 
   // batch of scenes (m_Batch); a single scene uses the reference-named kernels
   bool m_Batch = false;
@@ -189,7 +214,7 @@ class Particles {
   void* m_DParams = nullptr;          // FParams_t per scene
   void* m_DPBase = nullptr;
   void* m_DCBase = nullptr;
-  std::map<int, std::pair<void*, int>> m_BTab;  // threads per block -> (int2 table, blocks)
+  std::map<int, std::pair<void*, int>> m_BTab;  // particles per block -> (int2 table, entries)
 
   // adjoint (reverse mode)
   std::string m_CodePath;

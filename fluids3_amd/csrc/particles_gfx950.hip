@@ -38,6 +38,12 @@ __constant__ cuDataX FAccel;
 // the force stage tests its candidates in place.
 __constant__ int FNbrWords = FX_NBR_WORDS;
 __device__ __forceinline__ int nbr_cap() { return min(FNbrWords, FX_NBR_WORDS); }
+
+// FX_CAP_* bits of what the host set up beyond the reference's layout; 0 for
+// a host that binds this module as Fluids 5.0 binds its own.
+__constant__ int FHostCaps = 0;
+__device__ __forceinline__ bool host_tail() { return (FHostCaps & FX_CAP_TAIL) != 0; }
+__device__ __forceinline__ bool host_ranked() { return (FHostCaps & FX_CAP_RANKED) != 0; }
 #define FX_RANK_CAP 65536   // cells with more members keep atomic order
 
 // ---------------------------------------------------------------- helpers
@@ -66,11 +72,16 @@ struct Span {
 };
 
 // Batch lookup: entry b of the block table is (scene, block inside scene).
+// With a queue the launch is persistent: a fixed grid pulls the nitems
+// entries as work items (see batch_items).
 struct BatchArgs {
   const FParams_t* params;  // one FParams_t per scene
   const int* pbase;         // first particle of each scene (nscene + 1 entries)
   const int* cbase;         // first cell of each scene's table (nscene + 1)
-  const int2* btab;         // (scene, local block) per launched block
+  const int2* btab;         // (scene, local block) per block-table entry
+  uint* queue;              // FX_NUM_XCD claim counters, zero at launch; null: one block per entry
+  int nitems;               // entries of btab
+  int pad;
 };
 
 __device__ __forceinline__ Span batch_span(const BatchArgs& B, int b, const FParams_t*& P) {
@@ -144,6 +155,92 @@ __device__ __forceinline__ int xcd_block() {
   return ((k / FX_XCD_RUN) * FX_NUM_XCD + x) * FX_XCD_RUN + k % FX_XCD_RUN;
 }
 
+// The XCD this workgroup runs on (hardware register XCC_ID, bits 3:0).
+__device__ __forceinline__ int xcc_id() { return (int)(__builtin_amdgcn_s_getreg(20 | (3 << 11)) & 7u); }
+
+// Persistent launches.  A fixed grid of resident blocks pulls work items
+// (entries of the block table) until none is left.  The items are split into
+// FX_NUM_XCD contiguous ranges, one per XCD, each drained in order through
+// its own counter by the blocks running on that XCD, so that the blocks an
+// XCD holds at a time work on neighbouring particles, whose neighbour cells
+// then stay in that XCD's L2; a block whose range is exhausted takes items
+// from the following ranges in turn.  A block leaves once every range is
+// exhausted, so the launch drains whatever the grid size.  Thread 0 claims,
+// the block reads the item from LDS.
+// The claim state (current range, ranges tried) lives in LDS beside the
+// claimed item: only thread 0 uses it, and it stays out of the registers of
+// the work the block runs between claims.
+struct Claim {
+  int item, q, tried;
+};
+
+__device__ __forceinline__ int next_item(uint* ctr, int n, Claim* c) {
+  if (threadIdx.x == 0) {
+    int it = -1;
+    int q = c->q, tried = c->tried;
+    while (tried < FX_NUM_XCD) {
+      const int lo = (int)(((long long)n * q) / FX_NUM_XCD);
+      const int hi = (int)(((long long)n * (q + 1)) / FX_NUM_XCD);
+      const int k = (int)atomicAdd(&ctr[q], 1u);
+      if (k < hi - lo) {
+        it = lo + k;
+        break;
+      }
+      q = (q + 1) % FX_NUM_XCD;
+      tried++;
+    }
+    c->item = it;
+    c->q = q;
+    c->tried = tried;
+  }
+  __syncthreads();
+  const int it = c->item;
+  __syncthreads();  // every thread has read the item before thread 0 claims the next
+  return it;
+}
+
+// Runs body(entry) for the block-table entries this block processes: its own
+// entry (in XCD-aware order when xcd_order) for a one-block-per-entry launch,
+// or every entry it claims in a persistent one.
+template <class F>
+__device__ __forceinline__ void batch_items(const BatchArgs& B, bool xcd_order, F&& body) {
+  __shared__ Claim c;
+  const bool persistent = B.queue != nullptr;
+  if (persistent && threadIdx.x == 0) c = Claim{-1, xcc_id(), 0};
+  int it = persistent ? next_item(B.queue, B.nitems, &c) : (xcd_order ? xcd_block() : (int)blockIdx.x);
+  while (it >= 0) {  // one call site of the body
+    body(it);
+    it = persistent ? next_item(B.queue, B.nitems, &c) : -1;
+  }
+}
+
+// Persistent launches of the single-scene pair kernels: FPersistQ[stage] is
+// the device address of that stage's FX_NUM_XCD claim counters, or 0 for a
+// launch of one block per block of particles (the reference's launches leave
+// it 0).  The work items are then the blocks of particles, in XCD ranges.
+__constant__ uint64_t FPersistQ[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+template <class F>
+__device__ __forceinline__ void scene_items(int stage, int pnum, F&& body) {
+  uint* q = (uint*)FPersistQ[stage];
+  const BatchArgs B{nullptr, nullptr, nullptr, nullptr, q, (pnum + (int)blockDim.x - 1) / (int)blockDim.x, 0};
+  batch_items(B, true, body);
+}
+
+// Particles per thread of the streaming stages (index: FX_STAGE_INSERT 0,
+// FX_STAGE_SORT 2, FX_STAGE_ADVANCE 5): a block covers blockDim * ppt
+// consecutive particles, thread t taking t, t + blockDim, ...  A host that
+// does not set it runs one particle per thread, as the reference's launches
+// do (particles.cpp:26-30,294-296).
+#define FX_MAX_PPT 8
+__constant__ int FPpt[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+__device__ __forceinline__ int ppt_of(int stage) { return max(1, min(FPpt[stage], FX_MAX_PPT)); }
+// the k-th of a block's ppt sub-blocks of blockDim particles
+__device__ __forceinline__ Span sub_span(Span S, int ppt, int k) {
+  S.blk = S.blk * ppt + k;
+  return S;
+}
+
 // ----------------------------------------------------------- fill (memset)
 
 // Fill used instead of hipMemsetAsync so that a captured step graph holds
@@ -152,6 +249,10 @@ extern "C" __global__ void fxFillUint(uint* __restrict__ p, uint v, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+
+// No work: one dispatch of it marks a point of the stream in kernel traces
+// and counter runs (the benchmark brackets its timed region with two).
+extern "C" __global__ void fxMark(int tag) {}
 
 // ------------------------------------------------------------------ insert
 
@@ -163,15 +264,18 @@ extern "C" __global__ void fxFillUint(uint* __restrict__ p, uint v, int n) {
 // finite input and sends NaN to GRID_UNDEF.  Out-of-range particles are
 // counted in slot gridTotal of the scene (its tail bucket).  Stored cell
 // indices are offsets into the concatenated cell table (c0 + local cell).
+// Without a tail slot (tail = false: a table of gridTotal entries, as the
+// reference allocates) out-of-range particles are not counted and keep the
+// gndx the host cleared, as in particles.cu:45-52.
 //
 // The counting is aggregated per wave: the particles arrive sorted by cell
 // from the previous step, so the 64 lanes of a wave fall into few cells;
 // lanes of one cell take a single atomicAdd of their number and their slot
 // is the returned base plus their rank among those lanes.  The atomics of
 // all the wave's cells are issued together (no round trip per cell).
-__device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S) {
+__device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S, bool tail) {
   const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
-  const bool active = i < S.p1;
+  bool active = i < S.p1;  // below: particles that take a count slot
   int gs = 0;
   if (active) {
     const fx_float3 p = pbuf<const fx_float3>(FPnts, FPOS)[i];
@@ -186,6 +290,7 @@ __device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S) {
     } else {
       gs = S.c0 + P.gridTotal;
       pbuf<int>(FPnts, FGCELL)[i] = FX_GRID_UNDEF;
+      active = tail;
     }
     uint* tflag = pbuf<uint>(FAccel, ATAILFLAG);
     if (tflag != nullptr) tflag[i] = in ? 0u : 1u;
@@ -215,13 +320,18 @@ __device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S) {
 }
 
 extern "C" __global__ void insertParticles(int pnum) {
-  insert_impl(*cparams(&FParams), Span{0, pnum, 0, (int)blockIdx.x});
+  const int ppt = ppt_of(0);
+  for (int k = 0; k < ppt; k++)
+    insert_impl(*cparams(&FParams), sub_span(Span{0, pnum, 0, (int)blockIdx.x}, ppt, k), host_tail());
 }
 
 extern "C" __global__ void insertParticlesBatch(BatchArgs B) {
-  const FParams_t* P;
-  const Span S = batch_span(B, blockIdx.x, P);
-  insert_impl(*P, S);
+  const int ppt = ppt_of(0);
+  batch_items(B, false, [&](int b) {
+    const FParams_t* P;
+    const Span S = batch_span(B, b, P);
+    for (int k = 0; k < ppt; k++) insert_impl(*P, sub_span(S, ppt, k), true);
+  });
 }
 
 // ------------------------------------------------------------- prefix scan
@@ -366,39 +476,89 @@ __device__ __forceinline__ void sort_full_impl(const FParams_t& P, const Span S)
   }
   // every load before the first store (the two sets are distinct buffers,
   // which the compiler cannot know)
+  const bool has_id = FPntTmp.mbuf[FID] != 0 && FPnts.mbuf[FID] != 0;
   const fx_float3 pos = pbuf<const fx_float3>(FPntTmp, FPOS)[i];
   const fx_float3 vel = pbuf<const fx_float3>(FPntTmp, FVEL)[i];
   const fx_float3 vev = pbuf<const fx_float3>(FPntTmp, FVEVAL)[i];
   const uint clr = pbuf<const uint>(FPntTmp, FCLR)[i];
-  const uint id = pbuf<const uint>(FPntTmp, FID)[i];
+  const uint id = has_id ? pbuf<const uint>(FPntTmp, FID)[i] : 0u;
   const uint s = o + (uint)rank;
   pbuf<fx_float3>(FPnts, FPOS)[s] = pos;
   pbuf<fx_float3>(FPnts, FVEL)[s] = vel;
   pbuf<fx_float3>(FPnts, FVEVAL)[s] = vev;
   pbuf<uint>(FPnts, FCLR)[s] = clr;
-  pbuf<uint>(FPnts, FID)[s] = id;
+  if (has_id) pbuf<uint>(FPnts, FID)[s] = id;
   pbuf<int>(FPnts, FGCELL)[s] = c;
   pbuf<int>(FPnts, FGNDX)[s] = rank;
 }
 
+// The counting sort of a host that runs no countingSortIndex (the reference's
+// own, particles.cpp:832-842): every particle goes to off[cell] + gndx, its
+// insert slot, so the in-cell order is the insert's atomic order; the eight
+// reference buffers move (press and force included, particles.cu:80-89), the
+// id when both sets have one, and AGRID[slot] = slot (particles.cu:91).
+// Without a tail slot out-of-range particles are not moved (particles.cu:78),
+// with one they go to the scene's tail bucket in gndx order.
+__device__ __forceinline__ void sort_full_gndx(const FParams_t& P, const Span S, bool tail) {
+  const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
+  if (i >= S.p1) return;
+  const int c = pbuf<const int>(FPntTmp, FGCELL)[i];
+  if (c == FX_GRID_UNDEF && !tail) return;
+  const int cell = (c == FX_GRID_UNDEF) ? S.c0 + P.gridTotal : c;
+  const int nd = pbuf<const int>(FPntTmp, FGNDX)[i];
+  const uint s = pbuf<const uint>(FAccel, AGRIDOFF)[cell] + (uint)nd;
+  const bool has_id = FPntTmp.mbuf[FID] != 0 && FPnts.mbuf[FID] != 0;
+  const fx_float3 pos = pbuf<const fx_float3>(FPntTmp, FPOS)[i];
+  const fx_float3 vel = pbuf<const fx_float3>(FPntTmp, FVEL)[i];
+  const fx_float3 vev = pbuf<const fx_float3>(FPntTmp, FVEVAL)[i];
+  const fx_float3 frc = pbuf<const fx_float3>(FPntTmp, FFORCE)[i];
+  const float prs = pbuf<const float>(FPntTmp, FPRESS)[i];
+  const uint clr = pbuf<const uint>(FPntTmp, FCLR)[i];
+  const uint id = has_id ? pbuf<const uint>(FPntTmp, FID)[i] : 0u;
+  pbuf<fx_float3>(FPnts, FPOS)[s] = pos;
+  pbuf<fx_float3>(FPnts, FVEL)[s] = vel;
+  pbuf<fx_float3>(FPnts, FVEVAL)[s] = vev;
+  pbuf<fx_float3>(FPnts, FFORCE)[s] = frc;
+  pbuf<float>(FPnts, FPRESS)[s] = prs;
+  pbuf<uint>(FPnts, FCLR)[s] = clr;
+  if (has_id) pbuf<uint>(FPnts, FID)[s] = id;
+  pbuf<int>(FPnts, FGCELL)[s] = c;
+  pbuf<int>(FPnts, FGNDX)[s] = nd;
+  uint* grid = pbuf<uint>(FAccel, AGRID);
+  if (grid != nullptr) grid[s] = s;
+}
+
 extern "C" __global__ void countingSortIndex(int pnum) {
-  sort_index_impl(*cparams(&FParams), Span{0, pnum, 0, (int)blockIdx.x});
+  const int ppt = ppt_of(2);
+  for (int k = 0; k < ppt; k++) sort_index_impl(*cparams(&FParams), sub_span(Span{0, pnum, 0, (int)blockIdx.x}, ppt, k));
 }
 
 extern "C" __global__ void countingSortFull(int pnum) {
-  sort_full_impl(*cparams(&FParams), Span{0, pnum, 0, xcd_block()});
+  const int ppt = ppt_of(2);
+  for (int k = 0; k < ppt; k++) {
+    if (host_ranked())
+      sort_full_impl(*cparams(&FParams), sub_span(Span{0, pnum, 0, xcd_block()}, ppt, k));
+    else
+      sort_full_gndx(*cparams(&FParams), sub_span(Span{0, pnum, 0, (int)blockIdx.x}, ppt, k), host_tail());
+  }
 }
 
 extern "C" __global__ void countingSortIndexBatch(BatchArgs B) {
-  const FParams_t* P;
-  const Span S = batch_span(B, blockIdx.x, P);
-  sort_index_impl(*P, S);
+  const int ppt = ppt_of(2);
+  batch_items(B, false, [&](int b) {
+    const FParams_t* P;
+    const Span S = batch_span(B, b, P);
+    for (int k = 0; k < ppt; k++) sort_index_impl(*P, sub_span(S, ppt, k));
+  });
 }
 
 extern "C" __global__ void countingSortFullBatch(BatchArgs B) {
-  const FParams_t* P;
-  const Span S = batch_span(B, xcd_block(), P);
-  sort_full_impl(*P, S);
+  const int ppt = ppt_of(2);
+  batch_items(B, true, [&](int b) {
+    const FParams_t* P;
+    const Span S = batch_span(B, b, P);
+    for (int k = 0; k < ppt; k++) sort_full_impl(*P, sub_span(S, ppt, k));
+  });
 }
 
 // ----------------------------------------------------------------- density
@@ -440,8 +600,11 @@ struct BlockSpan {
 
 __device__ __forceinline__ bool block_span(const FParams_t& P, const Span S, int h0,
                                            const int* __restrict__ gcell, const uint* __restrict__ off,
-                                           BlockSpan& s) {
-  const int nvalid = (int)off[S.c0 + P.gridTotal];  // start of the scene's out-of-range tail
+                                           const uint* __restrict__ cnt, bool tail, BlockSpan& s) {
+  // end of the scene's in-range particles: the start of its tail bucket, or
+  // the end of its last cell when the table has no tail slot
+  const int last = S.c0 + P.gridTotal - 1;
+  const int nvalid = tail ? (int)off[last + 1] : (int)(off[last] + cnt[last]);
   if (h0 >= nvalid || h0 >= S.p1) return false;
   s.h1 = min(min(h0 + (int)blockDim.x, nvalid), S.p1);
   s.cf = gcell[h0];
@@ -473,7 +636,7 @@ __device__ __forceinline__ int nbr_words(const FParams_t& P, int gc, const uint*
 // at [w * N + i], N the size of the whole particle buffer) and the particle's
 // (pos, rho), (veval, 0) record.  A particle whose candidates need more than
 // FX_NBR_WORDS words records no bits.
-__device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, int N, float4* tile) {
+__device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, int N, float4* tile, bool tail) {
   const int t = threadIdx.x;
   const int h0 = S.p0 + S.blk * (int)blockDim.x;
   const int i = h0 + t;
@@ -484,7 +647,7 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
   uint* __restrict__ bits = pbuf<uint>(FAccel, ANBRBITS);
   const bool wbits = bits != nullptr && FAccel.mbuf[AFREC] != 0;
   BlockSpan B;
-  if (!block_span(P, S, h0, gcell, off, B)) return;  // uniform per block
+  if (!block_span(P, S, h0, gcell, off, cnt, tail, B)) return;  // uniform per block
   const bool act = i < B.h1;
   const int gc = act ? gcell[i] : B.cf;
   fx_float3 p = fx_float3{0.0f, 0.0f, 0.0f};
@@ -619,14 +782,20 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
 
 extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
   __shared__ float4 tile[TILE_P];
-  pressure_impl(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, tile);
+  scene_items(3, pnum, [&](int b) {
+    pressure_impl(*cparams(&FParams), Span{0, pnum, 0, b}, pnum, tile, host_tail());
+    __syncthreads();  // the next item's tile loads wait for this item's last reads
+  });
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computePressureBatch(BatchArgs B, int ntotal) {
   __shared__ float4 tile[TILE_P];
-  const FParams_t* P;
-  const Span S = batch_span(B, xcd_block(), P);
-  pressure_impl(*P, S, ntotal, tile);
+  batch_items(B, true, [&](int b) {
+    const FParams_t* P;
+    const Span S = batch_span(B, b, P);
+    pressure_impl(*P, S, ntotal, tile, true);
+    __syncthreads();  // the next item's tile loads wait for this item's last reads
+  });
 }
 
 // ------------------------------------------------------------------- force
@@ -1051,20 +1220,21 @@ __device__ __forceinline__ void force_q(const FParams_t& P, const Span S, int N,
     force_q_impl<2>(P, S, N, lq);
 }
 
-#ifndef FW_ATTR
-#define FW_ATTR
-#endif
-
-extern "C" __global__ __launch_bounds__(1024) FW_ATTR void computeForce(int pnum) {
+extern "C" __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) void computeForce(int pnum) {
   __shared__ uint lq[FW_QBYTES / 4];
-  force_q(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, lq);
+  scene_items(4, pnum, [&](int b) { force_q(*cparams(&FParams), Span{0, pnum, 0, b}, pnum, lq); });
 }
 
-extern "C" __global__ __launch_bounds__(1024) FW_ATTR void computeForceBatch(BatchArgs B, int ntotal) {
+// The persistent loop costs the batch kernel registers; held at the single
+// scene kernel's 5 waves per SIMD (96 VGPRs, no spills).
+extern "C" __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) void computeForceBatch(
+    BatchArgs B, int ntotal) {
   __shared__ uint lq[FW_QBYTES / 4];
-  const FParams_t* P;
-  const Span S = batch_span(B, xcd_block(), P);
-  force_q(*P, S, ntotal, lq);
+  batch_items(B, true, [&](int b) {
+    const FParams_t* P;
+    const Span S = batch_span(B, b, P);
+    force_q(*P, S, ntotal, lq);
+  });
 }
 
 // ------------------------------------------------------ neighbour statistics
@@ -1341,9 +1511,11 @@ __device__ __forceinline__ void advance_one(const FParams_t& P, int i, float tim
 }
 
 extern "C" __global__ void advanceParticles(float time, float dt, float ss, int numPnts) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= numPnts) return;
-  advance_one(*cparams(&FParams), i, time, dt, ss);
+  const int ppt = ppt_of(5);
+  for (int k = 0; k < ppt; k++) {
+    const int i = (blockIdx.x * ppt + k) * blockDim.x + threadIdx.x;
+    if (i < numPnts) advance_one(*cparams(&FParams), i, time, dt, ss);
+  }
 }
 
 // Same integrator with the simulation time read from device memory, for
@@ -1351,9 +1523,11 @@ extern "C" __global__ void advanceParticles(float time, float dt, float ss, int 
 // fxClockTick advances that clock by dt (the host's AdvanceTime,
 // particles.cpp:396-400).
 extern "C" __global__ void advanceParticlesClock(const float* clock, float dt, float ss, int numPnts) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= numPnts) return;
-  advance_one(*cparams(&FParams), i, clock[0], dt, ss);
+  const int ppt = ppt_of(5);
+  for (int k = 0; k < ppt; k++) {
+    const int i = (blockIdx.x * ppt + k) * blockDim.x + threadIdx.x;
+    if (i < numPnts) advance_one(*cparams(&FParams), i, clock[0], dt, ss);
+  }
 }
 
 extern "C" __global__ void fxClockTick(float* clock, float dt) {
@@ -1363,9 +1537,13 @@ extern "C" __global__ void fxClockTick(float* clock, float dt) {
 // Batch integrator: the time comes from clock[0] when clock is given, else
 // from the argument; dt and the scale from each scene's parameters.
 extern "C" __global__ void advanceParticlesBatch(BatchArgs B, const float* clock, float time) {
-  const FParams_t* P;
-  const Span S = batch_span(B, blockIdx.x, P);
-  const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
-  if (i >= S.p1) return;
-  advance_one(*P, i, clock != nullptr ? clock[0] : time, P->dt, P->sim_scale);
+  const int ppt = ppt_of(5);
+  batch_items(B, false, [&](int b) {
+    const FParams_t* P;
+    const Span S = batch_span(B, b, P);
+    for (int k = 0; k < ppt; k++) {
+      const int i = S.p0 + (S.blk * ppt + k) * (int)blockDim.x + (int)threadIdx.x;
+      if (i < S.p1) advance_one(*P, i, clock != nullptr ? clock[0] : time, P->dt, P->sim_scale);
+    }
+  });
 }

@@ -47,6 +47,7 @@ SIGNATURES = {
     "fx_stage": (C.c_int, [_P, C.c_int32]),
     "fx_run": (C.c_int, [_P, C.c_int32]),
     "fx_synchronize": (C.c_int, [_P]),
+    "fx_mark": (C.c_int, [_P, C.c_int32]),
     "fx_scan": (C.c_int, [_P, _P, _P, C.c_int32]),
     "fx_graph_capture": (C.c_int, [_P]),
     "fx_graph_launch": (C.c_int, [_P, C.c_int32]),
@@ -58,6 +59,9 @@ SIGNATURES = {
     "fx_set_stage_block_size": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "fx_stage_block_size": (C.c_int32, [_P, C.c_int32]),
     "fx_autotune": (C.c_int, [_P, C.c_int32, _P, _P]),
+    "fx_set_stage_ppt": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "fx_stage_ppt": (C.c_int32, [_P, C.c_int32]),
+    "fx_set_persistent": (C.c_int, [_P, C.c_int32]),
     "fx_adjoint_configure": (C.c_int, [_P, C.c_char_p, C.c_int32, C.c_int32]),
     "fx_adjoint_set_target": (C.c_int, [_P, _P, C.c_int64]),
     "fx_adjoint_forward": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -66,6 +70,8 @@ SIGNATURES = {
     "fx_adjoint_memory": (C.c_int64, [_P]),
     "fx_adjoint_segment": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "fx_emit_particles": (C.c_int, [_P, C.c_float, C.c_int32]),
+    "fx_scene_from_xml": (C.c_int, [C.c_char_p, C.c_int32, _P, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32]),
+    "fx_scene_xml_ids": (C.c_int, [C.c_char_p, _P, C.c_int32, C.POINTER(C.c_int32)]),
     "fx_last_error": (C.c_char_p, []),
     "fx_version": (C.c_char_p, []),
 }
@@ -223,6 +229,10 @@ class Particles:
     def Synchronize(self):
         check(lib().fx_synchronize(self._h), "fx_synchronize")
 
+    def Mark(self, tag):
+        """One dispatch of the no-op kernel fxMark(tag), marking this point of the stream in traces."""
+        check(lib().fx_mark(self._h, int(tag)), "fx_mark")
+
     def Scan(self, values):
         """Exclusive prefix sum of a uint32 array with the grid-offset scan kernels."""
         a = np.ascontiguousarray(values, dtype=np.uint32)
@@ -306,6 +316,16 @@ class Particles:
         st = (C.c_double * abi.NUM_STAGES)()
         check(lib().fx_bench_steps(self._h, nsteps, 1 if use_graph else 0, C.byref(tot), st), "fx_bench_steps")
         return tot.value, {abi.STAGE_NAMES[k]: st[k] for k in range(abi.NUM_STAGES)}
+
+    def SetStagePpt(self, stage, ppt):
+        check(lib().fx_set_stage_ppt(self._h, stage, ppt), "fx_set_stage_ppt")
+
+    def StagePpt(self, stage):
+        return lib().fx_stage_ppt(self._h, stage)
+
+    def SetPersistent(self, batch=True, scene=False):
+        """Persistent (work-queue) launches for batch stages and for single-scene density/force."""
+        check(lib().fx_set_persistent(self._h, (1 if batch else 0) | (2 if scene else 0)), "fx_set_persistent")
 
     def CountPairs(self):
         """(candidates, pairs) summed over particles on the current cell-sorted state."""

@@ -82,6 +82,26 @@ float fx_scene_spacing(const fx_scene* scene);
  * number of particles written. */
 int fx_scene_fill(const fx_scene* scene, float* pos, uint32_t* clr, int32_t n);
 
+/* Scene files in the reference's scene.xml schema (README.md "Scene
+ * parameters", fluids5.0/assets/scene.xml): a <Fluid> section, then
+ * <Scene id="k"> sections; a scene's keys override the Fluid section's, as
+ * FluidSystem::ParseXML applies "Fluid" and then the scene
+ * (fluids3.2/fluids/fluid_system.cpp:2268-2309).  text is the file's
+ * contents, NUL-terminated.
+ * fx_scene_from_xml: *scene holds the defaults on entry (fields the file does
+ * not set keep them) and the scene on return.  name (name_len bytes) receives
+ * <Name>; unused (unused_len bytes) receives "Key=value\n" for each key of a
+ * force or viewer setting the step does not have (PointGravAmt, PointGravPos,
+ * WaveForceMax, SearchDist, Draw*), or FX_ERANGE if it does not fit; either
+ * may be NULL.  FX_EINVAL for malformed text, an unknown key, a value that
+ * is not a number or three numbers, a missing scene or a <Num> that is not
+ * positive (*scene unchanged then).
+ * fx_scene_xml_ids: the ids of the <Scene> sections in file order (count of
+ * them in *count; FX_ERANGE if more than max_ids). */
+int fx_scene_from_xml(const char* text, int32_t scene_id, fx_scene* scene, char* name, int32_t name_len,
+                      char* unused, int32_t unused_len);
+int fx_scene_xml_ids(const char* text, int32_t* ids, int32_t max_ids, int32_t* count);
+
 /* ---- lifecycle ---- */
 
 /* Particles::Initialize (particles.cpp:68-93): loads the code object by path,
@@ -153,6 +173,11 @@ int fx_stage(fx_system* sys, int32_t stage);
 /* Particles::Run n times (stages in order, then AdvanceTime). */
 int fx_run(fx_system* sys, int32_t nsteps);
 int fx_synchronize(fx_system* sys);
+/* Enqueues one dispatch of the no-op kernel fxMark(tag): a point of the
+ * stream that kernel traces and counter runs show (no reference
+ * counterpart; the reference marks host time with PERF_PUSH/POP,
+ * fluids5.0/src/particles.cpp:384-389). */
+int fx_mark(fx_system* sys, int32_t tag);
 
 /* Exclusive prefix sum of n uints with the grid-offset scan kernels
  * (prefixSum / prefixFixup, up to three levels), through temporary device
@@ -208,14 +233,34 @@ int fx_emit_particles(fx_system* sys, float frame, int32_t emit);
 int fx_set_block_size(fx_system* sys, int32_t threads);
 int fx_set_stage_block_size(fx_system* sys, int32_t stage, int32_t threads);
 int32_t fx_stage_block_size(const fx_system* sys, int32_t stage);
+/* Particles per thread (1..8) of the streaming stages FX_STAGE_INSERT,
+ * FX_STAGE_SORT and FX_STAGE_ADVANCE: a block then covers threads * ppt
+ * consecutive particles (other stages: 1 only, FX_EINVAL otherwise).  The
+ * code object reads it from its module constant FPpt, which a host that
+ * does not set it leaves at 1.  Results do not depend on it. */
+int fx_set_stage_ppt(fx_system* sys, int32_t stage, int32_t ppt);
+int32_t fx_stage_ppt(const fx_system* sys, int32_t stage);
+/* Persistent launches: one launch of as many blocks as the device holds at
+ * once, pulling work items from per-XCD queues (each XCD drains a contiguous
+ * range of items, then helps the others), instead of one block per item.
+ * mode bits: FX_PERSIST_BATCH, every stage of a batch (fx_restart_batch),
+ * items = (scene, block); FX_PERSIST_SCENE, the density and force stages of
+ * a single scene, items = blocks of particles.  Default FX_PERSIST_BATCH.
+ * Results do not depend on it. */
+#define FX_PERSIST_BATCH 1
+#define FX_PERSIST_SCENE 2
+int fx_set_persistent(fx_system* sys, int32_t mode);
 
 /* Autotuner (no reference counterpart; the reference launches everything with
  * 512 threads per block, particles.cpp:417-418): profiles nsteps steps from
- * the current state for each block size in {64, 128, 256, 512, 1024}, keeps
- * the fastest per stage, checks that a step under the chosen sizes is
- * bit-identical to a step under the previous ones (FX_EMISMATCH otherwise),
- * and leaves the system at the state it started from with the chosen sizes.
- * threads_out / ms_out (optional): FX_NUM_STAGES entries each. */
+ * the current state for each block size in {64, 128, 256, 512, 1024} at one
+ * particle per thread and keeps the fastest per stage, then, at those sizes,
+ * 2 and 4 particles per thread for the streaming stages (fx_set_stage_ppt),
+ * keeping what beats one; checks that a step under the chosen configuration
+ * is bit-identical to a step under the previous one (FX_EMISMATCH
+ * otherwise), and leaves the system at the state it started from with the
+ * chosen configuration.  threads_out / ms_out (optional): FX_NUM_STAGES
+ * entries each; the particles per thread chosen: fx_stage_ppt. */
 int fx_autotune(fx_system* sys, int32_t nsteps, int32_t* threads_out, double* ms_out);
 
 /* ---- reverse mode (no reference counterpart: the reference has no

@@ -71,6 +71,15 @@ extern "C" {
 #define FX_NUM_ABUF 15
 #define FX_NBR_WORDS 256 /* bitmask words per particle (8192 candidates) */
 
+/* Module constant FHostCaps (int, default 0): what the host allocated and
+ * runs beyond the reference's layout.  A host that binds the code object the
+ * way Fluids 5.0 does (particles.cpp:267-300,596-626,716-919) leaves it 0:
+ * AGRIDCNT/AGRIDOFF hold gridTotal entries, out-of-range particles are not
+ * counted, and countingSortFull ranks every particle by its insert slot
+ * (gndx) and moves the reference's eight buffers.  libfluidsx sets both bits. */
+#define FX_CAP_TAIL 1   /* AGRIDCNT/AGRIDOFF hold gridTotal+1 entries (tail bucket) */
+#define FX_CAP_RANKED 2 /* countingSortIndex runs before countingSortFull (stable in-cell order) */
+
 #define FX_DMAXBUF 16 /* datax.h:37 */
 
 typedef struct fx_float3 {

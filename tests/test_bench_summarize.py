@@ -1,0 +1,54 @@
+"""bench.py --summarize on synthetic rocprofv3 CSVs: the timed window is the
+dispatches between the two fxMark dispatches, whatever runs before and after
+it, and the traffic per launch is the mean scaled counter over that window."""
+import csv
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def write_csv(path, rows, fields):
+    with open(path, "w", newline="") as fh:
+        w = csv.DictWriter(fh, fieldnames=fields)
+        w.writeheader()
+        w.writerows(rows)
+
+
+def test_summarize_cuts_the_window_between_the_marks(tmp_path):
+    # 3 force dispatches before the window (300 us), 2 inside (100 us), 1 after (900 us)
+    seq = [("computeForce", 300)] * 3 + [("fxMark", 1), ("computeForce", 100), ("computePressure", 50),
+                                         ("computeForce", 100), ("computePressure", 50), ("fxMark", 1),
+                                         ("computeForce", 900)]
+    trace, fetch, write = [], [], []
+    t = 0
+    for i, (k, us) in enumerate(seq):
+        trace.append({"Dispatch_Id": i + 1, "Kernel_Name": k + "(int)", "Start_Timestamp": t,
+                      "End_Timestamp": t + us * 1000})
+        t += us * 1000 + 10
+        inside = 3 < i < 8
+        fetch.append({"Dispatch_Id": i + 1, "Kernel_Name": k, "Counter_Name": "FETCH_SIZE",
+                      "Counter_Value": 1000.0 if inside else 99999.0})
+        write.append({"Dispatch_Id": i + 1, "Kernel_Name": k, "Counter_Name": "WRITE_SIZE",
+                      "Counter_Value": 10.0 if inside else 99999.0})
+    f1 = ["Dispatch_Id", "Kernel_Name", "Start_Timestamp", "End_Timestamp"]
+    f2 = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"]
+    write_csv(tmp_path / "kt.csv", trace, f1)
+    write_csv(tmp_path / "f.csv", fetch, f2)
+    write_csv(tmp_path / "w.csv", write, f2)
+    line = {"value": 1.0, "ms_per_step": 0.15, "config": {"workload": "dambreak_4m: x", "timed_window": {
+        "steps": 2, "warmup": 5, "autotuned": True, "graph": True}}}
+    (tmp_path / "line.json").write_text("noise\n" + json.dumps(line) + "\n")
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--summarize", "--line",
+                          str(tmp_path / "line.json"), "--trace", str(tmp_path / "kt.csv"), "--fetch",
+                          str(tmp_path / "f.csv"), "--write", str(tmp_path / "w.csv"), "--json",
+                          str(tmp_path / "t.json")], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert "| computeForce | 2 | 100.0 |" in out.stdout
+    t = json.loads((tmp_path / "t.json").read_text())
+    assert t["window"] == {"steps": 2, "warmup": 5, "autotuned": True, "graph": True}
+    assert t["window_avg_us"]["computeForce"] == 100.0
+    # FETCH x2 + WRITE x1, KiB -> bytes
+    assert t["bytes_per_launch"]["computeForce"] == (1000.0 * 2 + 10.0) * 1024

@@ -34,7 +34,11 @@ def test_bench_line_contract():
     assert abs(d["value"] - 16384 / (d["ms_per_step"] * 1e-3)) <= 1e-3 * d["value"]
     assert d["config"]["particles"] == 16384 and d["config"]["workload"].startswith("dambreak_16k")
     r = d["roofline"]
-    assert r["bound"] in ("hbm", "mfma") and r["peak"] > 0 and 0 < r["frac"] < 1
+    assert r["bound"] in ("hbm", "valu") and r["peak"] > 0 and 0 < r["frac"] < 1
     assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-3
     c = d["cpu_baseline"]
     assert c["value"] > 0 and c["cores"] >= 1 and c["kind"] in ("port", "reference")
+    assert "single_thread" in c and c["single_thread"]["cores"] == 1 and c["single_thread"]["value"] > 0
+    assert c["dambreak_16k_single_thread"]["value"] > 0
+    w = d["config"]["timed_window"]
+    assert (w["steps"], w["warmup"], w["autotuned"]) == (7, 3, False)

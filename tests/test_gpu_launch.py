@@ -1,0 +1,75 @@
+"""GPU: launch configurations do not change a bit of the results.
+
+- particles per thread 2, 4 and 8 for the streaming stages (insert, sort,
+  advance), at two block sizes, on a particle count that leaves ragged last
+  blocks, against one particle per thread;
+- a batch of scenes run with persistent stage launches (a fixed grid pulling
+  (scene, block) items from per-XCD queues), with one block per item, and
+  with particles per thread 4, eagerly and as the captured graph.
+"""
+import numpy as np
+import pytest
+
+from fluids3_amd import Particles, abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+STREAMING = (abi.STAGE_NAMES.index("insert"), abi.STAGE_NAMES.index("sort"), abi.STAGE_NAMES.index("advance"))
+FIELDS = ("pos", "vel", "veval", "press", "force", "id", "gcell", "gndx", "cnt", "off", "clr")
+
+
+def run_cfg(ps, restart, threads, ppt, steps, graph=False):
+    for k in range(abi.NUM_STAGES):
+        ps.SetStageBlockSize(k, threads)
+        ps.SetStagePpt(k, ppt if k in STREAMING else 1)
+    restart()
+    if graph:
+        ps.CaptureGraph()
+        ps.LaunchGraph(steps // 2)
+    else:
+        ps.Run(steps)
+    return ps.state()
+
+
+def same(a, b, what):
+    """Byte-equal fields (press and force only where the particle is in range:
+    out-of-range slots are not written by the pair stages)."""
+    valid = b["gcell"] != abi.GRID_UNDEF
+    for k in FIELDS:
+        x, y = np.ascontiguousarray(a[k]), np.ascontiguousarray(b[k])
+        if k in ("press", "force"):
+            x, y = x[valid], y[valid]
+        assert x.dtype == y.dtype and x.shape == y.shape, (what, k)
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), (what, k)
+
+
+def test_particles_per_thread_is_bit_identical():
+    sc = scenes.dambreak_16k(n=10007)  # ragged last blocks at every size
+    with Particles() as ps:
+        restart = lambda: ps.Restart(sc)
+        ref = run_cfg(ps, restart, 256, 1, 5)
+        for threads in (64, 256):
+            for ppt in (2, 4, 8):
+                print(f"ppt {ppt} at {threads} threads", flush=True)
+                same(run_cfg(ps, restart, threads, ppt, 5), ref, (threads, ppt))
+        with pytest.raises(Exception):
+            ps.SetStagePpt(abi.STAGE_NAMES.index("force"), 2)  # pair stages take one particle per lane
+        # persistent single-scene density and force
+        ps.SetPersistent(batch=True, scene=True)
+        for threads in (64, 256):
+            print(f"persistent density and force at {threads} threads", flush=True)
+            same(run_cfg(ps, restart, threads, 1, 5), ref, ("persistent", threads))
+            same(run_cfg(ps, restart, threads, 1, 6, graph=True), run_cfg(ps, restart, threads, 1, 6), ("graph", threads))
+
+
+def test_batch_persistent_launch_is_bit_identical():
+    sc = scenes.batch_scenes(count=9, lo=500, hi=40000, seed=31)
+    sc[4].num_points = 1  # a one-particle scene
+    with Particles() as ps:
+        restart = lambda: ps.RestartBatch(sc)
+        ps.SetPersistent(batch=False)
+        ref = run_cfg(ps, restart, 256, 1, 6)
+        ps.SetPersistent(batch=True)
+        for threads, ppt, graph in ((256, 1, False), (64, 4, False), (256, 4, True), (1024, 2, True)):
+            print(f"persistent batch: {threads} threads, ppt {ppt}, graph {graph}", flush=True)
+            same(run_cfg(ps, restart, threads, ppt, 6, graph), ref, (threads, ppt, graph))

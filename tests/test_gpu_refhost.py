@@ -1,0 +1,168 @@
+"""The code object as a drop-in for the reference's own host.
+
+tests/refhost/refhost binds fluids3_amd/particles_gfx950.hsaco exactly as
+Fluids 5.0's Particles binds particles.ptx (fluids5.0/src/particles.cpp:68-93,
+267-300, 596-626, 716-919): the reference's kernel names and arguments, 512
+threads per block, count/offset tables of gridTotal entries, the eight
+particle buffers without an id buffer, CopyAllToTemp's device copies before
+the sort, the NULL stream.  It never writes FHostCaps, so the module runs as
+under that host: the in-cell order is the insert's atomic order and
+out-of-range particles are left out of the sort.
+
+Checks against the oracle on the same inputs (particles carry a unique colour,
+which the sort moves with them, so they can be matched), after one step:
+cell counts and offsets bit-exact, every cell's members as sets, gndx a
+permutation of each cell's slots, AGRID the identity, the cell of every
+particle bit-exact, density within 1e-5 (floored relative).
+
+Force and the integrated state are sums of pair terms that cancel heavily on
+these compressed lattices, so their FP32 value depends on the summation order
+(the atomic in-cell order) by more than 1e-5: the oracle itself, fed the same
+particles in another order, moves by 3e-4 to 9e-4 in force.  Their bar is
+therefore accuracy: against the FP64 build of the same formulas they must be
+no further than 1.5 times the oracle's own distance from it (measured ratios
+under reordered oracle runs: 0.7 to 1.25).  Under the stable in-cell order of
+libfluidsx the same kernels are bit-identical to the oracle (test_gpu_step.py).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fluids3_amd import derive_params, fill_points, scenes
+from fluids3_amd.particles import CODE_OBJECT
+from oracle import oracle
+import parity
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "refhost", "refhost")
+STRIDE = [12, 12, 4, 12, 12, 4, 4, 4]  # FPOS FVEL FCLR FVEVAL FFORCE FPRESS FGCELL FGNDX
+GRID_UNDEF = 2147483647
+
+pytestmark = pytest.mark.gpu
+
+
+def run_driver(tmp_path, P, pos, clr, nsteps):
+    if not os.path.exists(DRIVER):
+        pytest.fail(f"{DRIVER} is missing: build it with __graft_entry__.build()")
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        f.write(bytes(P))
+        f.write(np.int32(pos.shape[0]).tobytes())
+        f.write(np.ascontiguousarray(pos, np.float32).tobytes())
+        f.write(np.ascontiguousarray(clr, np.uint32).tobytes())
+    print(f"refhost: {pos.shape[0]} particles, {nsteps} step(s)", flush=True)
+    r = subprocess.run([DRIVER, CODE_OBJECT, str(fin), str(fout), str(nsteps)], capture_output=True, text=True,
+                       timeout=90)
+    assert r.returncode == 0, f"refhost failed ({r.returncode}): {r.stdout}{r.stderr}"
+    raw = np.fromfile(fout, np.uint8)
+    n, G = np.frombuffer(raw[:8], np.int32)
+    o = 8
+    bufs = []
+    for s in STRIDE:
+        bufs.append(raw[o:o + s * n])
+        o += s * n
+    cnt = raw[o:o + 4 * G].view(np.uint32); o += 4 * G
+    off = raw[o:o + 4 * G].view(np.uint32); o += 4 * G
+    grid = raw[o:o + 4 * n].view(np.uint32)
+    f3 = lambda b: b.view(np.float32).reshape(n, 3)
+    return {"pos": f3(bufs[0]), "vel": f3(bufs[1]), "clr": bufs[2].view(np.uint32), "veval": f3(bufs[3]),
+            "force": f3(bufs[4]), "press": bufs[5].view(np.float32), "gcell": bufs[6].view(np.int32),
+            "gndx": bufs[7].view(np.int32), "cnt": cnt, "off": off, "grid": grid, "G": int(G)}
+
+
+def tagged_scene(n=16384):
+    sc = scenes.dambreak_16k(n=n)
+    P = derive_params(sc)
+    pos, _ = fill_points(sc)
+    tags = np.arange(pos.shape[0], dtype=np.uint32)  # unique colour per particle
+    return P, pos, tags
+
+
+def match_by_tag(d, ref, nvalid):
+    where = np.empty(ref["clr"].shape[0], np.int64)
+    where[ref["clr"][:nvalid]] = np.arange(nvalid)
+    return where[d["clr"][:nvalid]]
+
+
+def check_structure(d, ref, nvalid):
+    """Integer outputs exact: counts, offsets, cells, membership, slots, AGRID."""
+    G = d["G"]
+    parity.assert_int_equal("cnt", d["cnt"], ref["cnt"][:G])
+    parity.assert_int_equal("off", d["off"], ref["off"][:G])
+    parity.assert_int_equal("gcell", d["gcell"][:nvalid], ref["gcell"][:nvalid])
+    assert np.array_equal(d["grid"][:nvalid], np.arange(nvalid, dtype=np.uint32)), "AGRID is not the identity"
+    # every cell holds the same particles (tags), and gndx numbers its slots
+    assert parity.cell_members_equal(d["clr"][:nvalid], ref["clr"][:nvalid], ref["off"][:G], ref["cnt"][:G])
+    seg = np.repeat(np.arange(G), ref["cnt"][:G].astype(np.int64))
+    slot = np.arange(nvalid) - ref["off"][:G][seg]
+    gsort = np.lexsort((d["gndx"][:nvalid], seg))
+    assert np.array_equal(d["gndx"][:nvalid][gsort], slot[np.lexsort((slot, seg))]), "gndx is not a slot permutation"
+
+
+def check_accuracy(d, ref, ref64, nvalid, density_tol=1e-5):
+    """Density against the oracle; the order-sensitive fields against FP64."""
+    k = match_by_tag(d, ref, nvalid)
+    out = {}
+    if density_tol is not None:
+        out["press vs oracle"] = parity.assert_close("press", d["press"][:nvalid], ref["press"][k], tol=density_tol)
+    for f in ("press", "force", "pos", "vel", "veval"):
+        e_gpu = parity.rel_err(d[f][:nvalid], ref64[f][k])
+        e_orc = parity.rel_err(ref[f][:nvalid], ref64[f][:nvalid])
+        out[f"{f} vs fp64 (gpu/oracle)"] = f"{e_gpu:.2e}/{e_orc:.2e}"
+        assert e_gpu <= 1.5 * e_orc + 1e-7, f"{f}: {e_gpu:.3e} from FP64, the oracle {e_orc:.3e}"
+    return out
+
+
+def oracle_runs(P, pos, tags, nsteps):
+    ref = oracle.run(P, oracle.make_state(P, pos, clr=tags), nsteps)
+    ref64 = oracle.run(P, oracle.make_state(P, pos, clr=tags, dtype=np.float64), nsteps, fp64=True)
+    return ref, ref64
+
+
+def test_reference_host_one_step(tmp_path):
+    P, pos, tags = tagged_scene()
+    d = run_driver(tmp_path, P, pos, tags, 1)
+    ref, ref64 = oracle_runs(P, pos, tags, 1)
+    assert ref["cnt"][P.gridTotal] == 0  # nothing out of range
+    check_structure(d, ref, pos.shape[0])
+    print("reference host, one step:", check_accuracy(d, ref, ref64, pos.shape[0]))
+
+
+def test_reference_host_five_steps(tmp_path):
+    """After five steps the trajectories of two summation orders have parted
+    slightly, so a particle near a cell face may sit in the neighbouring cell:
+    the structure is checked by its invariants and by the fraction of
+    particles whose cell differs from the oracle's."""
+    P, pos, tags = tagged_scene()
+    n = pos.shape[0]
+    d = run_driver(tmp_path, P, pos, tags, 5)
+    ref, ref64 = oracle_runs(P, pos, tags, 5)
+    G = d["G"]
+    cnt = d["cnt"].astype(np.int64)
+    assert cnt.sum() == n - ref["cnt"][G]
+    assert np.array_equal(d["off"].astype(np.int64), np.concatenate(([0], np.cumsum(cnt)[:-1])))
+    assert np.array_equal(np.sort(d["clr"]), np.arange(n, dtype=np.uint32))
+    assert np.all(np.diff(d["gcell"].astype(np.int64)) >= 0)
+    k = match_by_tag(d, ref, n)
+    moved = np.count_nonzero(d["gcell"] != ref["gcell"][k])
+    assert moved <= n // 1000, f"{moved} particles in another cell than the oracle's"
+    print(f"reference host, five steps ({moved} particles in another cell):",
+          check_accuracy(d, ref, ref64, n, density_tol=None))
+
+
+def test_reference_host_out_of_range_particle(tmp_path):
+    """A particle outside the grid: the reference leaves it out of the counts
+    and of the sort (particles.cu:45-52,78), so the in-range particles fill
+    slots [0, n-1) exactly as the oracle sorts them."""
+    P, pos, tags = tagged_scene(n=8192)
+    pos = pos.copy()
+    pos[100] = (500.0, 30.0, 0.0)
+    d = run_driver(tmp_path, P, pos, tags, 1)
+    ref, ref64 = oracle_runs(P, pos, tags, 1)
+    n = pos.shape[0]
+    assert ref["cnt"][P.gridTotal] == 1
+    check_structure(d, ref, n - 1)
+    assert 100 not in set(d["clr"][: n - 1].tolist())
+    print("reference host, out-of-range particle:", check_accuracy(d, ref, ref64, n - 1))

@@ -469,10 +469,12 @@ def test_autotune_keeps_state_and_results(ps):
         for k in ("pos", "vel", "veval", "id", "clr"):
             assert bits_equal(before[k], after[k]), k
         assert before["time"] == after["time"]
+        assert all(ps.StagePpt(k) in (1, 2, 4) for k in range(abi.NUM_STAGES))
         ps.Run(4)
         tuned = ps.state()
         for k, t in enumerate(defaults):
             ps.SetStageBlockSize(k, t)
+            ps.SetStagePpt(k, 1)
         ps.Restart(sc)
         ps.Run(7)
         plain = ps.state()
@@ -481,6 +483,7 @@ def test_autotune_keeps_state_and_results(ps):
     finally:
         for k, t in enumerate(defaults):
             ps.SetStageBlockSize(k, t)
+            ps.SetStagePpt(k, 1)
 
 
 def test_reference_default_scene(ps):

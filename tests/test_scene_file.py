@@ -64,3 +64,66 @@ def test_unused_forces_are_reported_and_errors_raised():
         load_scene(TEXT.replace("<GroundSlope>", "<Bogus>").replace("</GroundSlope>", "</Bogus>"), 1)
     with pytest.raises(SceneFileError):
         load_scene(TEXT.replace("-50, 0, -25", "-50, 0"), 1)
+
+
+# ------------------------------------------- the committed schema fixture
+
+import json
+import os
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def fixture_text():
+    with open(os.path.join(GOLDEN, "scene_schema.xml")) as fh:
+        return fh.read()
+
+
+def test_fixture_matches_hand_written_values():
+    """tests/golden/scene_schema.xml against values read off the file by hand:
+    the XML declaration and comments are skipped, an entity is decoded, the
+    scene's <Viscosity> overrides the Fluid section's, and keys of forces the
+    step lacks come back as unused."""
+    text = fixture_text()
+    with open(os.path.join(GOLDEN, "scene_schema_expected.json")) as fh:
+        want = json.load(fh)
+    assert scene_ids(text) == want["ids"]
+    for sid in want["ids"]:
+        name, s, unused = load_scene(text, sid)
+        exp = want[sid]
+        assert name == exp["name"]
+        assert unused == exp["unused"]
+        for k, v in exp.items():
+            if k in ("name", "unused"):
+                continue
+            got = getattr(s, k)
+            if isinstance(v, list):
+                assert got.tolist() == pytest.approx(v), (sid, k)
+            else:
+                assert got == pytest.approx(v, rel=1e-6), (sid, k)
+
+
+def test_truncated_and_malformed_files_are_refused_not_crashed():
+    """Every prefix of the fixture, and a set of broken variants, either parse
+    or raise SceneFileError (the reader is bounds-checked C++)."""
+    text = fixture_text()
+    for cut in range(0, len(text), 7):  # every 7th prefix
+        try:
+            load_scene(text[:cut], 7)
+        except SceneFileError:
+            pass
+    bad = [
+        text.replace("</Fluid>", ""),                         # unclosed section
+        text.replace("<DT>", "<DT a=1>"),                     # unquoted attribute
+        text.replace("0.0025", "0.0025x"),                    # not a number
+        text.replace("&amp;", "&bogus;"),                     # unknown entity
+        text.replace("<Num>           12000", "<Num> -5"),    # negative count
+        text.replace("</Name>", "</Nmae>"),                   # mismatched end tag
+        text.replace("<VolMin>        -40, 0, -20", "<VolMin> 1,2,3,4"),  # four components
+        "<Scene id=\"7\"><Num>10</Num><Inner><X>1</X></Inner></Scene>",  # nested element
+    ]
+    for t in bad:
+        with pytest.raises(SceneFileError):
+            load_scene(t, 7)
+    with pytest.raises(SceneFileError):
+        load_scene(text, "seven")
