@@ -224,7 +224,7 @@ def plain_bench(args, workload, K, W, extras=True):
 
     batch = workload == "batch64"
     ps = Particles() if args.code_object is None else Particles(args.code_object)
-    ps.SetPersistent(batch=args.persistent in ("batch", "both"), scene=args.persistent in ("scene", "both"))
+    ps.SetPersistent(args.persistent == "on")
     if args.stage_threads:
         for k, v in enumerate(args.stage_threads.split(",")):
             ps.SetStageBlockSize(k, int(v))
@@ -519,30 +519,31 @@ def summarize(args):
     win = {k: line["config"]["timed_window"][k] for k in ("steps", "warmup", "autotuned", "graph")}
     K = win["steps"]
     out = []
-    rows = _rows(args.trace)
-    a, b = _window(rows)
     dur = {}
-    whole = {}
-    for r in rows:
-        k = _base(r["Kernel_Name"])
-        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
-        whole.setdefault(k, []).append(d)
-        if a < int(r["Dispatch_Id"]) < b:
-            dur.setdefault(k, []).append(d)
-    tot = sum(sum(v) for v in whole.values())
-    out.append("Whole run (every dispatch of the command):\n")
-    out.append("| kernel | calls | avg us | min us | max us | share |\n|---|---:|---:|---:|---:|---:|")
-    for k, v in sorted(whole.items(), key=lambda kv: -sum(kv[1])):
-        if sum(v) / tot < 0.0005:
-            continue
-        out.append(f"| {k} | {len(v)} | {np.mean(v):.1f} | {min(v):.1f} | {max(v):.1f} | {100 * sum(v) / tot:.1f}% |")
-    out.append(f"\nTimed window (dispatches between fxMark(1) and fxMark(2), ids {a}..{b}; {K} steps):\n")
-    out.append("| kernel | dispatches | avg us | min us | max us |\n|---|---:|---:|---:|---:|")
-    for k, v in sorted(dur.items(), key=lambda kv: -sum(kv[1])):
-        out.append(f"| {k} | {len(v)} | {np.mean(v):.1f} | {min(v):.1f} | {max(v):.1f} |")
-    step_us = sum(sum(v) for v in dur.values()) / K
-    out.append(f"\nKernel time per step in the window: {step_us:.1f} us; the line's ms_per_step: "
-               f"{1000 * line['ms_per_step']:.1f} us (events, launch gaps included).")
+    if args.trace:
+        rows = _rows(args.trace)
+        a, b = _window(rows)
+        whole = {}
+        for r in rows:
+            k = _base(r["Kernel_Name"])
+            d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+            whole.setdefault(k, []).append(d)
+            if a < int(r["Dispatch_Id"]) < b:
+                dur.setdefault(k, []).append(d)
+        tot = sum(sum(v) for v in whole.values())
+        out.append("Whole run (every dispatch of the command):\n")
+        out.append("| kernel | calls | avg us | min us | max us | share |\n|---|---:|---:|---:|---:|---:|")
+        for k, v in sorted(whole.items(), key=lambda kv: -sum(kv[1])):
+            if sum(v) / tot < 0.0005:
+                continue
+            out.append(f"| {k} | {len(v)} | {np.mean(v):.1f} | {min(v):.1f} | {max(v):.1f} | {100 * sum(v) / tot:.1f}% |")
+        out.append(f"\nTimed window (dispatches between fxMark(1) and fxMark(2), ids {a}..{b}; {K} steps):\n")
+        out.append("| kernel | dispatches | avg us | min us | max us |\n|---|---:|---:|---:|---:|")
+        for k, v in sorted(dur.items(), key=lambda kv: -sum(kv[1])):
+            out.append(f"| {k} | {len(v)} | {np.mean(v):.1f} | {min(v):.1f} | {max(v):.1f} |")
+        step_us = sum(sum(v) for v in dur.values()) / K
+        out.append(f"\nKernel time per step in the window: {step_us:.1f} us; the line's ms_per_step: "
+                   f"{1000 * line['ms_per_step']:.1f} us (events, launch gaps included).")
     traffic = {}
     if args.fetch and args.write:
         per = {}
@@ -594,9 +595,8 @@ def main():
                     help="threads per block of the six stages, comma separated (insert,prefix,sort,pressure,force,"
                          "advance); empty = the library's defaults")
     ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly")
-    ap.add_argument("--persistent", default="batch", choices=["none", "batch", "scene", "both"],
-                    help="persistent (per-XCD work-queue) launches: every stage of a batch, the density and force "
-                         "stages of a single scene, both, or none")
+    ap.add_argument("--persistent", default="off", choices=["on", "off"],
+                    help="batch stages as persistent launches (per-XCD work queues) or one block per item")
     ap.add_argument("--stats-every", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="oracle steps timed for the all-core CPU baseline; 0 = as many as fill --cpu-seconds (at most 50)")
@@ -612,7 +612,7 @@ def main():
                          "counters of a kernel of known traffic) and exit")
     ap.add_argument("--summarize", action="store_true", help="reduce rocprofv3 CSVs of a bench.py run (see summarize)")
     ap.add_argument("--line", help="--summarize: the bench.py JSON line of the profiled command")
-    ap.add_argument("--trace", help="--summarize: kernel_trace.csv")
+    ap.add_argument("--trace", help="--summarize: kernel_trace.csv (optional)")
     ap.add_argument("--fetch", help="--summarize: counter_collection.csv of a FETCH_SIZE run")
     ap.add_argument("--write", help="--summarize: counter_collection.csv of a WRITE_SIZE run")
     ap.add_argument("--fetch-factor", type=float, default=2.0)

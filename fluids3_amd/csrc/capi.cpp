@@ -173,9 +173,9 @@ int32_t fx_stage_ppt(const fx_system* s, int32_t stage) {
   return s->p.StagePpt(stage);
 }
 
-int fx_set_persistent(fx_system* s, int32_t mode) {
+int fx_set_persistent(fx_system* s, int32_t on) {
   FX_CHECK_SYS(s);
-  return s->p.SetPersistent(mode);
+  return s->p.SetPersistent(on != 0);
 }
 
 int fx_mark(fx_system* s, int32_t tag) {

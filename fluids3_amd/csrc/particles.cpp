@@ -19,7 +19,9 @@ static const char* kKernelNames[FUNC_MAX] = {
     "fxFillUint",        "advanceParticlesClock", "fxClockTick",     "fxCountPairs",
     "insertParticlesBatch", "countingSortIndexBatch", "countingSortFullBatch", "computePressureBatch",
     "computeForceBatch", "advanceParticlesBatch", "fxCountPairsBatch", "sampleParticles",
-    "emitParticles",     "fxMaxWords",       "fxMaxWordsBatch",  "fxMark"};
+    "emitParticles",     "fxMaxWords",       "fxMaxWordsBatch",  "fxMark",
+    "insertParticlesBatchP", "countingSortIndexBatchP", "countingSortFullBatchP", "computePressureBatchP",
+    "computeForceBatchP", "advanceParticlesBatchP"};
 
 Particles::Particles() { memset(&m_Params, 0, sizeof(m_Params)); }
 
@@ -88,11 +90,7 @@ int Particles::Initialize(const char* path) {
     FX_HIP(hipGetDevice(&dev), "Initialize");
     FX_HIP(hipDeviceGetAttribute(&m_NumCU, hipDeviceAttributeMultiprocessorCount, dev), "Initialize");
   }
-  FX_HIP(hipModuleGetGlobal(reinterpret_cast<hipDeviceptr_t*>(&m_PersistSym), &len, m_Module, "FPersistQ"),
-         "Initialize");
-  if (len != 8 * sizeof(uint64_t)) return fx::fail(FX_EMISMATCH, "Initialize: FPersistQ has %zu bytes", len);
   FX_TRY(UploadPpt());
-  FX_TRY(UploadPersistQ());
   m_Params.numThreads = 256;
   return FX_OK;
 }
@@ -133,45 +131,11 @@ int Particles::SetStagePpt(int stage, int ppt) {
   return BuildBlockTables();
 }
 
-int Particles::SetPersistent(int mode) {
-  if (mode & ~(FX_PERSIST_BATCH | FX_PERSIST_SCENE)) return fx::fail(FX_EINVAL, "SetPersistent: mode %d", mode);
+int Particles::SetPersistent(bool on) {
+  if (on == m_Persistent) return FX_OK;
   ReleaseGraph();
-  m_Persistent = (mode & FX_PERSIST_BATCH) != 0;
-  m_ScenePersistent = (mode & FX_PERSIST_SCENE) != 0;
-  return UploadPersistQ();
-}
-
-int Particles::UploadPersistQ() {
-  // the single-scene density and force kernels find their claim counters here
-  uint64_t q[8] = {};
-  if (m_ScenePersistent) {
-    q[FX_STAGE_PRESSURE] = (uint64_t)(uintptr_t)(m_WorkQ + 8 * FX_STAGE_PRESSURE);
-    q[FX_STAGE_FORCE] = (uint64_t)(uintptr_t)(m_WorkQ + 8 * FX_STAGE_FORCE);
-  }
-  FX_HIP(hipStreamSynchronize(m_Stream ? m_Stream : nullptr), "persistent");
-  FX_HIP(hipMemcpyHtoD(reinterpret_cast<hipDeviceptr_t>(m_PersistSym), q, sizeof(q)), "persistent");
+  m_Persistent = on;
   return FX_OK;
-}
-
-int Particles::LaunchPair(int fid, int stage) {
-  // the single-scene density / force launch: one block per block of
-  // particles, or (persistent) as many as the device holds, after clearing
-  // the stage's claim counters
-  void* args[] = {&mNumPoints};
-  if (!m_ScenePersistent) return LaunchP(fid, stage, args);
-  const int waves = (mNumPoints + 63) / 64 * 64;
-  const int threads = m_StageThreads[stage] < waves ? m_StageThreads[stage] : waves;
-  if (threads <= 0) return FX_OK;
-  unsigned grid = (unsigned)((mNumPoints + threads - 1) / threads);
-  int resident = 0;
-  FX_TRY(ResidentBlocks(fid, threads, &resident));
-  if ((unsigned)resident < grid) grid = (unsigned)resident;
-  int eight = 8;
-  unsigned zero = 0;
-  void* q = m_WorkQ + 8 * stage;
-  void* fargs[] = {&q, &zero, &eight};
-  FX_TRY(Launch(FUNC_FILL, 1, 64, fargs));
-  return Launch(fid, grid, (unsigned)threads, args);
 }
 
 int Particles::ResidentBlocks(int fid, int threads, int* blocks) {
@@ -292,17 +256,18 @@ int Particles::BuildBlockTables() {
   return FX_OK;
 }
 
-int Particles::BatchLaunch(int fid, int stage, bool persistent, void* extra0, void* extra1, void* extra2) {
-  // One launch over every scene of the batch.  Non-persistent: one block per
-  // block-table entry.  Persistent: as many blocks as the device holds at
-  // once (capped by the entries), pulling entries from the stage's claim
-  // counters, which a fill kernel clears first.
+int Particles::BatchLaunch(int fid, int fid_p, int stage, void* extra0, void* extra1, void* extra2) {
+  // One launch over every scene of the batch: one block per block-table
+  // entry, or (persistent) as many blocks as the device holds at once,
+  // capped by the entries, pulling entries from the stage's claim counters,
+  // which a fill kernel clears first.
   const int T = m_StageThreads[stage];
   const auto it = m_BTab.find(T * m_StagePpt[stage]);
   if (it == m_BTab.end()) return fx::fail(FX_ESTATE, "batch launch: no block table for stage %d", stage);
   FxBatchArgs B{m_DParams, m_DPBase, m_DCBase, it->second.first, nullptr, it->second.second, 0};
   unsigned grid = (unsigned)it->second.second;
-  if (persistent && m_Persistent) {
+  if (fid_p > 0 && m_Persistent) {
+    fid = fid_p;
     B.queue = m_WorkQ + 8 * stage;
     int resident = 0;
     FX_TRY(ResidentBlocks(fid, T, &resident));
@@ -486,7 +451,7 @@ int Particles::InsertParticles() {
   void* fargs[] = {&cnt, &zero, &ncell};
   FX_TRY(Launch(FUNC_FILL, (unsigned)((ncell + 255) / 256), 256, fargs));
   if (m_Batch) {
-    FX_TRY(BatchLaunch(FUNC_INSERT_B, FX_STAGE_INSERT, true, nullptr, nullptr, nullptr));
+    FX_TRY(BatchLaunch(FUNC_INSERT_B, FUNC_INSERT_BP, FX_STAGE_INSERT, nullptr, nullptr, nullptr));
   } else {
     void* args[] = {&mNumPoints};
     FX_TRY(LaunchP(FUNC_INSERT, FX_STAGE_INSERT, args));
@@ -547,8 +512,8 @@ int Particles::CountingSort() {
   m_Parity ^= 1;
   FX_TRY(PublishTables());
   if (m_Batch) {
-    FX_TRY(BatchLaunch(FUNC_SORT_INDEX_B, FX_STAGE_SORT, true, nullptr, nullptr, nullptr));
-    FX_TRY(BatchLaunch(FUNC_SORT_FULL_B, FX_STAGE_SORT, true, nullptr, nullptr, nullptr));
+    FX_TRY(BatchLaunch(FUNC_SORT_INDEX_B, FUNC_SORT_INDEX_BP, FX_STAGE_SORT, nullptr, nullptr, nullptr));
+    FX_TRY(BatchLaunch(FUNC_SORT_FULL_B, FUNC_SORT_FULL_BP, FX_STAGE_SORT, nullptr, nullptr, nullptr));
   } else {
     void* args[] = {&mNumPoints};
     FX_TRY(LaunchP(FUNC_SORT_INDEX, FX_STAGE_SORT, args));
@@ -560,9 +525,10 @@ int Particles::CountingSort() {
 
 int Particles::ComputePressure() {
   if (m_Batch) {
-    FX_TRY(BatchLaunch(FUNC_PRESS_B, FX_STAGE_PRESSURE, true, &mNumPoints, nullptr, nullptr));
+    FX_TRY(BatchLaunch(FUNC_PRESS_B, FUNC_PRESS_BP, FX_STAGE_PRESSURE, &mNumPoints, nullptr, nullptr));
   } else {
-    FX_TRY(LaunchPair(FUNC_COMPUTE_PRESS, FX_STAGE_PRESSURE));
+    void* args[] = {&mNumPoints};
+    FX_TRY(LaunchP(FUNC_COMPUTE_PRESS, FX_STAGE_PRESSURE, args));
   }
   m_Done = FX_STAGE_PRESSURE;
   return FX_OK;
@@ -570,9 +536,10 @@ int Particles::ComputePressure() {
 
 int Particles::ComputeForce() {
   if (m_Batch) {
-    FX_TRY(BatchLaunch(FUNC_FORCE_B, FX_STAGE_FORCE, true, &mNumPoints, nullptr, nullptr));
+    FX_TRY(BatchLaunch(FUNC_FORCE_B, FUNC_FORCE_BP, FX_STAGE_FORCE, &mNumPoints, nullptr, nullptr));
   } else {
-    FX_TRY(LaunchPair(FUNC_COMPUTE_FORCE, FX_STAGE_FORCE));
+    void* args[] = {&mNumPoints};
+    FX_TRY(LaunchP(FUNC_COMPUTE_FORCE, FX_STAGE_FORCE, args));
   }
   m_Done = FX_STAGE_FORCE;
   return FX_OK;
@@ -585,7 +552,7 @@ int Particles::Advance() {
   if (m_Batch) {
     float time = m_Params.time;
     void* clk = m_Capturing ? (void*)m_Clock : nullptr;
-    FX_TRY(BatchLaunch(FUNC_ADVANCE_B, FX_STAGE_ADVANCE, true, &clk, &time, nullptr));
+    FX_TRY(BatchLaunch(FUNC_ADVANCE_B, FUNC_ADVANCE_BP, FX_STAGE_ADVANCE, &clk, &time, nullptr));
     if (!m_Capturing) return FX_OK;
     void* targs[] = {&m_Clock, &dt};
     return Launch(FUNC_CLOCK_TICK, 1, 64, targs);
@@ -876,7 +843,7 @@ int Particles::MaxNbrWords(int* words) {
   void* fargs[] = {&st, &zero, &four};
   FX_TRY(Launch(FUNC_FILL, 1, 64, fargs));
   if (m_Batch) {
-    FX_TRY(BatchLaunch(FUNC_MAXWORDS_B, FX_STAGE_PRESSURE, false, &st, nullptr, nullptr));
+    FX_TRY(BatchLaunch(FUNC_MAXWORDS_B, 0, FX_STAGE_PRESSURE, &st, nullptr, nullptr));
   } else {
     void* args[] = {&st, &mNumPoints};
     FX_TRY(LaunchP(FUNC_MAXWORDS, FX_STAGE_PRESSURE, args));
@@ -897,7 +864,7 @@ int Particles::CountPairs(int64_t* cand, int64_t* pairs) {
   void* fargs[] = {&st, &zero, &four};
   FX_TRY(Launch(FUNC_FILL, 1, 64, fargs));
   if (m_Batch) {
-    FX_TRY(BatchLaunch(FUNC_COUNT_PAIRS_B, FX_STAGE_PRESSURE, false, &st, nullptr, nullptr));
+    FX_TRY(BatchLaunch(FUNC_COUNT_PAIRS_B, 0, FX_STAGE_PRESSURE, &st, nullptr, nullptr));
   } else {
     void* args[] = {&st, &mNumPoints};
     FX_TRY(LaunchP(FUNC_COUNT_PAIRS, FX_STAGE_PRESSURE, args));

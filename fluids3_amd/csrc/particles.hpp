@@ -42,7 +42,13 @@
 #define FUNC_MAXWORDS 21
 #define FUNC_MAXWORDS_B 22
 #define FUNC_MARK 23
-#define FUNC_MAX 24
+#define FUNC_INSERT_BP 24  // persistent batch kernels
+#define FUNC_SORT_INDEX_BP 25
+#define FUNC_SORT_FULL_BP 26
+#define FUNC_PRESS_BP 27
+#define FUNC_FORCE_BP 28
+#define FUNC_ADVANCE_BP 29
+#define FUNC_MAX 30
 
 #include <map>
 #include <string>
@@ -86,9 +92,9 @@ class Particles {
   // particles per thread of the insert, sort and advance stages (1..8)
   int SetStagePpt(int stage, int ppt);
   int StagePpt(int stage) const { return m_StagePpt[stage]; }
-  // persistent (fixed-grid, work-queue) launches: FX_PERSIST_* bits
-  int SetPersistent(int mode);
-  int Persistent() const { return (m_Persistent ? FX_PERSIST_BATCH : 0) | (m_ScenePersistent ? FX_PERSIST_SCENE : 0); }
+  // persistent (fixed-grid, work-queue) launches of the batch stages
+  int SetPersistent(bool on);
+  bool Persistent() const { return m_Persistent; }
 
   // the step (particles.cpp:382-400)
   int Run();
@@ -148,13 +154,12 @@ class Particles {
   int PublishTables();
   void ReleaseGraph();
   int BuildBlockTables();
-  // kernel arguments and grid of a batch launch of function fid for a stage;
-  // persistent launches reset the stage's claim counters first
-  int BatchLaunch(int fid, int stage, bool persistent, void* extra0, void* extra1, void* extra2);
+  // one batch launch of a stage: kernel fid, one block per block-table
+  // entry, or its persistent variant fid_p (0: none) after resetting the
+  // stage's claim counters
+  int BatchLaunch(int fid, int fid_p, int stage, void* extra0, void* extra1, void* extra2);
   int ResidentBlocks(int fid, int threads, int* blocks);
   int UploadPpt();
-  int UploadPersistQ();
-  int LaunchPair(int fid, int stage);
   int FinishRestart(int n);
   // adjoint helpers
   void AdjRelease();
@@ -199,9 +204,7 @@ class Particles {
   int m_StageThreads[FX_NUM_STAGES] = {128, 256, 512, 256, 256, 64};
   int m_StagePpt[FX_NUM_STAGES] = {1, 1, 1, 1, 1, 1};
   void* m_PptSym = nullptr;     // module constant FPpt
-  bool m_Persistent = true;     // batch kernels run as persistent launches
-  bool m_ScenePersistent = false;  // single-scene density and force run as persistent launches
-  void* m_PersistSym = nullptr;    // module constant FPersistQ
+  bool m_Persistent = false;    // batch stages run as persistent launches
   uint32_t* m_WorkQ = nullptr;  // 8 claim counters per stage
   int m_NumCU = 0;
   std::map<std::pair<int, int>, int> m_Resident;  // (function, threads) -> resident blocks on the device

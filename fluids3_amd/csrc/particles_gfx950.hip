@@ -199,32 +199,13 @@ __device__ __forceinline__ int next_item(uint* ctr, int n, Claim* c) {
   return it;
 }
 
-// Runs body(entry) for the block-table entries this block processes: its own
-// entry (in XCD-aware order when xcd_order) for a one-block-per-entry launch,
-// or every entry it claims in a persistent one.
+// Runs body(entry) for every block-table entry this block claims in a
+// persistent launch.
 template <class F>
-__device__ __forceinline__ void batch_items(const BatchArgs& B, bool xcd_order, F&& body) {
+__device__ __forceinline__ void batch_items(const BatchArgs& B, F&& body) {
   __shared__ Claim c;
-  const bool persistent = B.queue != nullptr;
-  if (persistent && threadIdx.x == 0) c = Claim{-1, xcc_id(), 0};
-  int it = persistent ? next_item(B.queue, B.nitems, &c) : (xcd_order ? xcd_block() : (int)blockIdx.x);
-  while (it >= 0) {  // one call site of the body
-    body(it);
-    it = persistent ? next_item(B.queue, B.nitems, &c) : -1;
-  }
-}
-
-// Persistent launches of the single-scene pair kernels: FPersistQ[stage] is
-// the device address of that stage's FX_NUM_XCD claim counters, or 0 for a
-// launch of one block per block of particles (the reference's launches leave
-// it 0).  The work items are then the blocks of particles, in XCD ranges.
-__constant__ uint64_t FPersistQ[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-
-template <class F>
-__device__ __forceinline__ void scene_items(int stage, int pnum, F&& body) {
-  uint* q = (uint*)FPersistQ[stage];
-  const BatchArgs B{nullptr, nullptr, nullptr, nullptr, q, (pnum + (int)blockDim.x - 1) / (int)blockDim.x, 0};
-  batch_items(B, true, body);
+  if (threadIdx.x == 0) c = Claim{-1, xcc_id(), 0};
+  for (int it = next_item(B.queue, B.nitems, &c); it >= 0; it = next_item(B.queue, B.nitems, &c)) body(it);
 }
 
 // Particles per thread of the streaming stages (index: FX_STAGE_INSERT 0,
@@ -319,19 +300,31 @@ __device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S, bo
   if (active) pbuf<int>(FPnts, FGNDX)[i] = (int)(base + lanes_below(mine));
 }
 
+// The streaming kernels keep a straight-line body for one particle per
+// thread (the reference's launches and the usual choice), the loop otherwise.
 extern "C" __global__ void insertParticles(int pnum) {
   const int ppt = ppt_of(0);
-  for (int k = 0; k < ppt; k++)
-    insert_impl(*cparams(&FParams), sub_span(Span{0, pnum, 0, (int)blockIdx.x}, ppt, k), host_tail());
+  const Span S{0, pnum, 0, (int)blockIdx.x};
+  if (ppt == 1) {
+    insert_impl(*cparams(&FParams), S, host_tail());
+    return;
+  }
+  for (int k = 0; k < ppt; k++) insert_impl(*cparams(&FParams), sub_span(S, ppt, k), host_tail());
 }
 
-extern "C" __global__ void insertParticlesBatch(BatchArgs B) {
+// one block-table entry of a batch stage (the *Batch kernels run one per
+// block, the *BatchP kernels claim them)
+__device__ __forceinline__ void insert_entry(const BatchArgs& B, int b) {
   const int ppt = ppt_of(0);
-  batch_items(B, false, [&](int b) {
-    const FParams_t* P;
-    const Span S = batch_span(B, b, P);
-    for (int k = 0; k < ppt; k++) insert_impl(*P, sub_span(S, ppt, k), true);
-  });
+  const FParams_t* P;
+  const Span S = batch_span(B, b, P);
+  for (int k = 0; k < ppt; k++) insert_impl(*P, sub_span(S, ppt, k), true);
+}
+
+extern "C" __global__ void insertParticlesBatch(BatchArgs B) { insert_entry(B, blockIdx.x); }
+
+extern "C" __global__ void insertParticlesBatchP(BatchArgs B) {
+  batch_items(B, [&](int b) { insert_entry(B, b); });
 }
 
 // ------------------------------------------------------------- prefix scan
@@ -530,35 +523,56 @@ __device__ __forceinline__ void sort_full_gndx(const FParams_t& P, const Span S,
 
 extern "C" __global__ void countingSortIndex(int pnum) {
   const int ppt = ppt_of(2);
-  for (int k = 0; k < ppt; k++) sort_index_impl(*cparams(&FParams), sub_span(Span{0, pnum, 0, (int)blockIdx.x}, ppt, k));
+  const Span S{0, pnum, 0, (int)blockIdx.x};
+  if (ppt == 1) {
+    sort_index_impl(*cparams(&FParams), S);
+    return;
+  }
+  for (int k = 0; k < ppt; k++) sort_index_impl(*cparams(&FParams), sub_span(S, ppt, k));
 }
 
 extern "C" __global__ void countingSortFull(int pnum) {
   const int ppt = ppt_of(2);
-  for (int k = 0; k < ppt; k++) {
-    if (host_ranked())
-      sort_full_impl(*cparams(&FParams), sub_span(Span{0, pnum, 0, xcd_block()}, ppt, k));
-    else
+  if (!host_ranked()) {  // a host without the index sort: the insert's slots
+    for (int k = 0; k < ppt; k++)
       sort_full_gndx(*cparams(&FParams), sub_span(Span{0, pnum, 0, (int)blockIdx.x}, ppt, k), host_tail());
+    return;
   }
+  const Span S{0, pnum, 0, xcd_block()};
+  if (ppt == 1) {
+    sort_full_impl(*cparams(&FParams), S);
+    return;
+  }
+  for (int k = 0; k < ppt; k++) sort_full_impl(*cparams(&FParams), sub_span(S, ppt, k));
 }
 
-extern "C" __global__ void countingSortIndexBatch(BatchArgs B) {
+__device__ __forceinline__ void sort_index_entry(const BatchArgs& B, int b) {
   const int ppt = ppt_of(2);
-  batch_items(B, false, [&](int b) {
-    const FParams_t* P;
-    const Span S = batch_span(B, b, P);
-    for (int k = 0; k < ppt; k++) sort_index_impl(*P, sub_span(S, ppt, k));
-  });
+  const FParams_t* P;
+  const Span S = batch_span(B, b, P);
+  for (int k = 0; k < ppt; k++) sort_index_impl(*P, sub_span(S, ppt, k));
 }
 
-extern "C" __global__ void countingSortFullBatch(BatchArgs B) {
+__device__ __forceinline__ void sort_full_entry(const BatchArgs& B, int b) {
   const int ppt = ppt_of(2);
-  batch_items(B, true, [&](int b) {
-    const FParams_t* P;
-    const Span S = batch_span(B, b, P);
-    for (int k = 0; k < ppt; k++) sort_full_impl(*P, sub_span(S, ppt, k));
-  });
+  const FParams_t* P;
+  const Span S = batch_span(B, b, P);
+  if (ppt == 1) {
+    sort_full_impl(*P, S);
+    return;
+  }
+  for (int k = 0; k < ppt; k++) sort_full_impl(*P, sub_span(S, ppt, k));
+}
+
+extern "C" __global__ void countingSortIndexBatch(BatchArgs B) { sort_index_entry(B, blockIdx.x); }
+extern "C" __global__ void countingSortFullBatch(BatchArgs B) { sort_full_entry(B, xcd_block()); }
+
+extern "C" __global__ void countingSortIndexBatchP(BatchArgs B) {
+  batch_items(B, [&](int b) { sort_index_entry(B, b); });
+}
+
+extern "C" __global__ void countingSortFullBatchP(BatchArgs B) {
+  batch_items(B, [&](int b) { sort_full_entry(B, b); });
 }
 
 // ----------------------------------------------------------------- density
@@ -782,19 +796,23 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
 
 extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
   __shared__ float4 tile[TILE_P];
-  scene_items(3, pnum, [&](int b) {
-    pressure_impl(*cparams(&FParams), Span{0, pnum, 0, b}, pnum, tile, host_tail());
-    __syncthreads();  // the next item's tile loads wait for this item's last reads
-  });
+  pressure_impl(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, tile, host_tail());
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computePressureBatch(BatchArgs B, int ntotal) {
   __shared__ float4 tile[TILE_P];
-  batch_items(B, true, [&](int b) {
+  const FParams_t* P;
+  const Span S = batch_span(B, xcd_block(), P);
+  pressure_impl(*P, S, ntotal, tile, true);
+}
+
+extern "C" __global__ __launch_bounds__(1024) void computePressureBatchP(BatchArgs B, int ntotal) {
+  __shared__ float4 tile[TILE_P];
+  batch_items(B, [&](int b) {
     const FParams_t* P;
     const Span S = batch_span(B, b, P);
     pressure_impl(*P, S, ntotal, tile, true);
-    __syncthreads();  // the next item's tile loads wait for this item's last reads
+    __syncthreads();  // the next entry's tile loads wait for this entry's last reads
   });
 }
 
@@ -1220,17 +1238,24 @@ __device__ __forceinline__ void force_q(const FParams_t& P, const Span S, int N,
     force_q_impl<2>(P, S, N, lq);
 }
 
-extern "C" __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) void computeForce(int pnum) {
+extern "C" __global__ __launch_bounds__(1024) void computeForce(int pnum) {
   __shared__ uint lq[FW_QBYTES / 4];
-  scene_items(4, pnum, [&](int b) { force_q(*cparams(&FParams), Span{0, pnum, 0, b}, pnum, lq); });
+  force_q(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, lq);
 }
 
-// The persistent loop costs the batch kernel registers; held at the single
-// scene kernel's 5 waves per SIMD (96 VGPRs, no spills).
-extern "C" __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) void computeForceBatch(
+extern "C" __global__ __launch_bounds__(1024) void computeForceBatch(BatchArgs B, int ntotal) {
+  __shared__ uint lq[FW_QBYTES / 4];
+  const FParams_t* P;
+  const Span S = batch_span(B, xcd_block(), P);
+  force_q(*P, S, ntotal, lq);
+}
+
+// The persistent loop costs the kernel registers (101 VGPRs, 4 waves per
+// SIMD); held at the one-block-per-entry kernel's 5 waves (96, no spills).
+extern "C" __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(5))) void computeForceBatchP(
     BatchArgs B, int ntotal) {
   __shared__ uint lq[FW_QBYTES / 4];
-  batch_items(B, true, [&](int b) {
+  batch_items(B, [&](int b) {
     const FParams_t* P;
     const Span S = batch_span(B, b, P);
     force_q(*P, S, ntotal, lq);
@@ -1512,6 +1537,11 @@ __device__ __forceinline__ void advance_one(const FParams_t& P, int i, float tim
 
 extern "C" __global__ void advanceParticles(float time, float dt, float ss, int numPnts) {
   const int ppt = ppt_of(5);
+  if (ppt == 1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < numPnts) advance_one(*cparams(&FParams), i, time, dt, ss);
+    return;
+  }
   for (int k = 0; k < ppt; k++) {
     const int i = (blockIdx.x * ppt + k) * blockDim.x + threadIdx.x;
     if (i < numPnts) advance_one(*cparams(&FParams), i, time, dt, ss);
@@ -1524,6 +1554,11 @@ extern "C" __global__ void advanceParticles(float time, float dt, float ss, int 
 // particles.cpp:396-400).
 extern "C" __global__ void advanceParticlesClock(const float* clock, float dt, float ss, int numPnts) {
   const int ppt = ppt_of(5);
+  if (ppt == 1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < numPnts) advance_one(*cparams(&FParams), i, clock[0], dt, ss);
+    return;
+  }
   for (int k = 0; k < ppt; k++) {
     const int i = (blockIdx.x * ppt + k) * blockDim.x + threadIdx.x;
     if (i < numPnts) advance_one(*cparams(&FParams), i, clock[0], dt, ss);
@@ -1536,14 +1571,20 @@ extern "C" __global__ void fxClockTick(float* clock, float dt) {
 
 // Batch integrator: the time comes from clock[0] when clock is given, else
 // from the argument; dt and the scale from each scene's parameters.
-extern "C" __global__ void advanceParticlesBatch(BatchArgs B, const float* clock, float time) {
+__device__ __forceinline__ void advance_entry(const BatchArgs& B, int b, const float* clock, float time) {
   const int ppt = ppt_of(5);
-  batch_items(B, false, [&](int b) {
-    const FParams_t* P;
-    const Span S = batch_span(B, b, P);
-    for (int k = 0; k < ppt; k++) {
-      const int i = S.p0 + (S.blk * ppt + k) * (int)blockDim.x + (int)threadIdx.x;
-      if (i < S.p1) advance_one(*P, i, clock != nullptr ? clock[0] : time, P->dt, P->sim_scale);
-    }
-  });
+  const FParams_t* P;
+  const Span S = batch_span(B, b, P);
+  for (int k = 0; k < ppt; k++) {
+    const int i = S.p0 + (S.blk * ppt + k) * (int)blockDim.x + (int)threadIdx.x;
+    if (i < S.p1) advance_one(*P, i, clock != nullptr ? clock[0] : time, P->dt, P->sim_scale);
+  }
+}
+
+extern "C" __global__ void advanceParticlesBatch(BatchArgs B, const float* clock, float time) {
+  advance_entry(B, blockIdx.x, clock, time);
+}
+
+extern "C" __global__ void advanceParticlesBatchP(BatchArgs B, const float* clock, float time) {
+  batch_items(B, [&](int b) { advance_entry(B, b, clock, time); });
 }

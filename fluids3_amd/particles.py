@@ -323,9 +323,9 @@ class Particles:
     def StagePpt(self, stage):
         return lib().fx_stage_ppt(self._h, stage)
 
-    def SetPersistent(self, batch=True, scene=False):
-        """Persistent (work-queue) launches for batch stages and for single-scene density/force."""
-        check(lib().fx_set_persistent(self._h, (1 if batch else 0) | (2 if scene else 0)), "fx_set_persistent")
+    def SetPersistent(self, on=True):
+        """Persistent (work-queue) launches of the batch stages."""
+        check(lib().fx_set_persistent(self._h, 1 if on else 0), "fx_set_persistent")
 
     def CountPairs(self):
         """(candidates, pairs) summed over particles on the current cell-sorted state."""

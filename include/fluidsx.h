@@ -240,16 +240,13 @@ int32_t fx_stage_block_size(const fx_system* sys, int32_t stage);
  * does not set it leaves at 1.  Results do not depend on it. */
 int fx_set_stage_ppt(fx_system* sys, int32_t stage, int32_t ppt);
 int32_t fx_stage_ppt(const fx_system* sys, int32_t stage);
-/* Persistent launches: one launch of as many blocks as the device holds at
- * once, pulling work items from per-XCD queues (each XCD drains a contiguous
- * range of items, then helps the others), instead of one block per item.
- * mode bits: FX_PERSIST_BATCH, every stage of a batch (fx_restart_batch),
- * items = (scene, block); FX_PERSIST_SCENE, the density and force stages of
- * a single scene, items = blocks of particles.  Default FX_PERSIST_BATCH.
+/* Batch stage launches (fx_restart_batch): persistent (1) runs each stage as
+ * one launch of as many blocks as the device holds at once, pulling (scene,
+ * block) work items from per-XCD queues (each XCD drains a contiguous range
+ * of items, then helps the others); 0 (the default, faster on MI355X: the
+ * claims cost more than the hardware dispatcher) launches one block per item.
  * Results do not depend on it. */
-#define FX_PERSIST_BATCH 1
-#define FX_PERSIST_SCENE 2
-int fx_set_persistent(fx_system* sys, int32_t mode);
+int fx_set_persistent(fx_system* sys, int32_t on);
 
 /* Autotuner (no reference counterpart; the reference launches everything with
  * 512 threads per block, particles.cpp:417-418): profiles nsteps steps from

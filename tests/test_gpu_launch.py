@@ -54,12 +54,7 @@ def test_particles_per_thread_is_bit_identical():
                 same(run_cfg(ps, restart, threads, ppt, 5), ref, (threads, ppt))
         with pytest.raises(Exception):
             ps.SetStagePpt(abi.STAGE_NAMES.index("force"), 2)  # pair stages take one particle per lane
-        # persistent single-scene density and force
-        ps.SetPersistent(batch=True, scene=True)
-        for threads in (64, 256):
-            print(f"persistent density and force at {threads} threads", flush=True)
-            same(run_cfg(ps, restart, threads, 1, 5), ref, ("persistent", threads))
-            same(run_cfg(ps, restart, threads, 1, 6, graph=True), run_cfg(ps, restart, threads, 1, 6), ("graph", threads))
+
 
 
 def test_batch_persistent_launch_is_bit_identical():
@@ -67,9 +62,9 @@ def test_batch_persistent_launch_is_bit_identical():
     sc[4].num_points = 1  # a one-particle scene
     with Particles() as ps:
         restart = lambda: ps.RestartBatch(sc)
-        ps.SetPersistent(batch=False)
+        ps.SetPersistent(False)
         ref = run_cfg(ps, restart, 256, 1, 6)
-        ps.SetPersistent(batch=True)
+        ps.SetPersistent(True)
         for threads, ppt, graph in ((256, 1, False), (64, 4, False), (256, 4, True), (1024, 2, True)):
             print(f"persistent batch: {threads} threads, ppt {ppt}, graph {graph}", flush=True)
             same(run_cfg(ps, restart, threads, ppt, 6, graph), ref, (threads, ppt, graph))

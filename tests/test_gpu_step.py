@@ -450,6 +450,18 @@ def test_dambreak_4m_full_size(ps):
     gf = ps.state()
     assert_close("force", gf["force"][m], r["force"][m])
     assert bits_equal(gf["force"][m], r["force"][m])
+    # error against the FP64 build of the same formulas on the same sorted state,
+    # per result: density, then force from the FP64 densities
+    r64 = oracle.run(p, snapshot_to_oracle(g, np.float64), 1, oracle.STAGE_PRESSURE | oracle.STAGE_FORCE,
+                     threads=THREADS, fp64=True)
+    for k, a in (("press", gp["press"]), ("force", gf["force"])):
+        a, b = np.asarray(a[m], np.float64).ravel(), np.asarray(r64[k][m], np.float64).ravel()
+        floor = 1e-3 * np.sqrt(np.mean(b * b))
+        e = np.abs(a - b) / np.maximum(np.abs(b), floor)
+        nw = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"4M FP32 vs FP64 {k}: floored max {e.max():.2e} p99 {np.quantile(e, 0.99):.2e} "
+              f"median {np.median(e):.2e} norm-wise {nw:.2e}")
+        assert e.max() < (1e-5 if k == "press" else 1e-1), k
 
 
 def test_autotune_keeps_state_and_results(ps):
