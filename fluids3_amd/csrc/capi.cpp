@@ -183,6 +183,11 @@ int fx_mark(fx_system* s, int32_t tag) {
   return s->p.Mark(tag);
 }
 
+int fx_scan_device_noalloc(fx_system* s, const uint32_t* in, uint32_t* out, int32_t n) {
+  FX_CHECK_SYS(s);
+  return s->p.ScanDeviceNoAlloc(in, out, n);
+}
+
 int fx_graph_capture(fx_system* s) {
   FX_CHECK_SYS(s);
   return s->p.CaptureGraph();

@@ -9,6 +9,7 @@
 
 #include "fx_common.hpp"
 #include "fx_scene.hpp"
+#include "noalloc.hpp"
 
 #define SCAN_BLOCKSIZE 512
 #define SCAN_ELEMS (2 * SCAN_BLOCKSIZE)
@@ -91,6 +92,7 @@ int Particles::Initialize(const char* path) {
     FX_HIP(hipDeviceGetAttribute(&m_NumCU, hipDeviceAttributeMultiprocessorCount, dev), "Initialize");
   }
   FX_TRY(UploadPpt());
+  FX_TRY(fx::scratch_setup(&m_ScanScratch));
   m_Params.numThreads = 256;
   return FX_OK;
 }
@@ -746,6 +748,23 @@ int Particles::Scan(const uint32_t* in, uint32_t* out, int n) {
   for (auto p : b)
     if (p) (void)hipFree(p);
   return st;
+}
+
+int Particles::ScanDeviceNoAlloc(const uint32_t* in, uint32_t* out, int n) {
+  // Scan on the caller's device buffers with the upper levels in the
+  // library's scratch array: no allocation, no host copy.
+  if (m_Module == nullptr || m_ScanScratch == nullptr)
+    return fx::fail(FX_ESTATE, "scan_device_noalloc: call Initialize first");
+  if (n <= 0 || in == nullptr || out == nullptr || (const void*)in == (const void*)out)
+    return fx::fail(FX_EINVAL, "scan_device_noalloc: bad arguments");
+  if (n > FX_SCAN_NOALLOC_MAX)
+    return fx::fail(FX_ERANGE, "scan_device_noalloc: %d elements exceed its scratch (at most %d)", n,
+                    FX_SCAN_NOALLOC_MAX);
+  uint32_t* s = static_cast<uint32_t*>(m_ScanScratch);
+  FX_TRY(ScanLevels(const_cast<uint32_t*>(in), out, s, s + FX_SCAN_L2, s + 2 * FX_SCAN_L2,
+                    s + 2 * FX_SCAN_L2 + FX_SCAN_L3, n));
+  FX_HIP(hipStreamSynchronize(m_Stream), "scan_device_noalloc");
+  return FX_OK;
 }
 
 int Particles::Mark(int tag) {

@@ -119,6 +119,7 @@ class Particles {
   int DownloadAccel(int buf, void* host, int64_t count);
   int Synchronize();
   int Scan(const uint32_t* in, uint32_t* out, int n);
+  int ScanDeviceNoAlloc(const uint32_t* in, uint32_t* out, int n);
   int Mark(int tag);
 
   // timing
@@ -192,6 +193,7 @@ class Particles {
   int m_Done = -1;
   float* m_Clock = nullptr;
   void* m_Stats = nullptr;  // two uint64 counters of fxCountPairs
+  void* m_ScanScratch = nullptr;  // this device's scratch array (noalloc.hip), not owned
   void* m_NbrWordsSym = nullptr;  // the module's FNbrWords (bitmask words a particle may record)
 
   hipGraph_t m_Graph = nullptr;

@@ -49,6 +49,7 @@ SIGNATURES = {
     "fx_synchronize": (C.c_int, [_P]),
     "fx_mark": (C.c_int, [_P, C.c_int32]),
     "fx_scan": (C.c_int, [_P, _P, _P, C.c_int32]),
+    "fx_scan_device_noalloc": (C.c_int, [_P, _P, _P, C.c_int32]),
     "fx_graph_capture": (C.c_int, [_P]),
     "fx_graph_launch": (C.c_int, [_P, C.c_int32]),
     "fx_profile_stages": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double)]),

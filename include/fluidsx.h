@@ -185,6 +185,17 @@ int fx_mark(fx_system* sys, int32_t tag);
  * n <= 1024^3 - 1. */
 int fx_scan(fx_system* sys, const uint32_t* in, uint32_t* out, int32_t n);
 
+/* The same exclusive scan on device buffers, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  in and out are
+ * distinct device buffers of n uint32 on the system's device; the caller's
+ * writes to in must be complete.  The upper scan levels use a fixed device
+ * array of the library (about 1 MB per GPU, held until the process ends), so
+ * n <= FX_SCAN_NOALLOC_MAX (else FX_ERANGE), and calls that overlap on one GPU
+ * race.  Returns when out is written. */
+#define FX_SCAN_NOALLOC_MAX (1 << 27)
+int fx_scan_device_noalloc(fx_system* sys, const uint32_t* in, uint32_t* out, int32_t n);
+
 /* Captures two consecutive steps (the device sets alternate roles each step)
  * into a HIP graph; fx_graph_launch replays it, advancing 2*nlaunch steps.
  * The simulation time then lives on the device. */
