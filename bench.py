@@ -144,6 +144,11 @@ def count_pairs_run(ps, restart, W, K, every):
     return csum / (samples * N), psum / (samples * N)
 
 
+def progress(msg):
+    """One line per phase on stderr (the JSON line alone goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def host_cpu():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -219,7 +224,7 @@ def timed_steps(ps, K, use_graph):
     return ms_total, stage_ms
 
 
-def plain_bench(args, workload, K, W, extras=True):
+def plain_bench(args, workload, K, W, extras=True, cpu=True):
     from fluids3_amd import Particles, scenes
 
     batch = workload == "batch64"
@@ -239,6 +244,7 @@ def plain_bench(args, workload, K, W, extras=True):
     restart()
     N, G = ps.NumPoints(), ps.CellTableSize() - ps.NumScenes()
     p = ps.params
+    progress(f"{workload}: {N:,} particles, {W} warm-up + {K} timed steps")
 
     # warm-up, block-size tuning on the warmed-up state (the tuner restores
     # it), then the timed region (one graph of K steps with per-stage events;
@@ -263,10 +269,11 @@ def plain_bench(args, workload, K, W, extras=True):
         stage_ms = dict(stage_ms)
     thr = [ps.StageBlockSize(k) for k in range(6)]
     value = N * K / (ms_total * 1e-3)
+    progress(f"{workload}: timed region {ms_total:.2f} ms; counting candidates and pairs")
     ms_step = ms_total / K
 
     # same trajectory again (bit-reproducible), untimed, for C and P
-    want_cpu = not args.no_cpu_baseline and not batch
+    want_cpu = cpu and not args.no_cpu_baseline and not batch
     if want_cpu:
         restart()
         ps.Run(W)
@@ -329,6 +336,7 @@ def plain_bench(args, workload, K, W, extras=True):
 
     if extras and workload in CONFIGURED and (W, K) != CONFIGURED[workload]:
         cw, ck = CONFIGURED[workload]
+        progress(f"{workload}: the configured window, {cw} + {ck} steps")
         restart()
         ps.Run(cw)
         cms, cstage = timed_steps(ps, ck, use_graph)
@@ -343,10 +351,12 @@ def plain_bench(args, workload, K, W, extras=True):
 
     # forward+adjoint rollout beside the plain step, same start state and window
     if workload == "dambreak_4m" and args.adjoint_steps > 0:
+        progress(f"{workload}: forward+adjoint of {args.adjoint_steps} steps")
         seg = args.checkpoint_every if args.checkpoint_every is not None else 0
         out["adjoint"] = adjoint_ratio(ps, restart, W, args.adjoint_steps, seg, use_graph)
 
     if want_cpu:
+        progress(f"{workload}: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(p, start_state, N, workload, args)
     ps.close()
     return out
@@ -470,12 +480,20 @@ def extras(args):
     """Summaries of the other configurations, carried in the default line."""
     ex = {}
     t0 = time.time()
-    b = plain_bench(args, "batch64", 100, args.warmup, extras=False)
+    progress("extras: ref_beach")
+    r = plain_bench(args, "ref_beach", 200, 20, extras=False, cpu=False)
+    ex["ref_beach"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "steps": 200, "warmup": 20,
+                       "particles": r["config"]["particles"], "vs_rtx4090_published": r["vs_baseline"],
+                       "published": "160 fps = 6.4e8 particle-steps/s on an RTX 4090 (README.md:12-19)",
+                       "roofline": {k: r["roofline"][k] for k in ("bound", "frac", "kernel", "kernel_ms")}}
+    progress("extras: batch64")
+    b = plain_bench(args, "batch64", 100, args.warmup, extras=False, cpu=False)
     ex["batch64"] = {"value": b["value"], "ms_per_step": b["ms_per_step"], "steps": 100, "warmup": args.warmup,
                      "particles": b["config"]["particles"],
                      "single_scene": b["config"].get("single_scene"),
                      "roofline": {k: b["roofline"][k] for k in ("bound", "frac", "kernel", "kernel_ms")}}
     for w in ("adjoint_262k", "adjoint_batch32"):
+        progress(f"extras: {w}")
         a = adjoint_bench(args, w, auto=False)
         c = a["config"]
         ex[w] = {"value": a["value"], "steps": a["steps"], "checkpoint_every": c["checkpoint_every"],
@@ -564,6 +582,22 @@ def summarize(args):
             f, w = np.mean(v["FETCH_SIZE"]), np.mean(v["WRITE_SIZE"])
             traffic[k] = (f + w) * 1024.0
             out.append(f"| {k} | {len(v['FETCH_SIZE'])} | {f:.0f} | {w:.0f} | {traffic[k] / 1e6:.1f} |")
+    if args.counters:
+        # any other counters: mean per dispatch over the window, per kernel
+        per = {}
+        for path in args.counters:
+            rr = _rows(path)
+            a2, b2 = _window(rr)
+            for r in rr:
+                if a2 < int(r["Dispatch_Id"]) < b2:
+                    per.setdefault(_base(r["Kernel_Name"]), {}).setdefault(r["Counter_Name"], []).append(
+                        float(r["Counter_Value"]))
+        names = sorted({c for v in per.values() for c in v})
+        out.append("\nCounters per dispatch in the window (mean):\n")
+        out.append("| kernel | dispatches | " + " | ".join(names) + " |\n|---|---:|" + "---:|" * len(names))
+        for k, v in sorted(per.items()):
+            nd = max(len(x) for x in v.values())
+            out.append(f"| {k} | {nd} | " + " | ".join(f"{np.mean(v[c]):.4g}" if c in v else "" for c in names) + " |")
     print("\n".join(out))
     if args.json:
         with open(args.json, "w") as fh:
@@ -618,6 +652,8 @@ def main():
     ap.add_argument("--fetch-factor", type=float, default=2.0)
     ap.add_argument("--write-factor", type=float, default=1.0)
     ap.add_argument("--json", help="--summarize: traffic summary to write")
+    ap.add_argument("--counters", nargs="*", default=[],
+                    help="--summarize: counter_collection.csv files of other --pmc runs of the same command")
     args = ap.parse_args()
 
     if args.summarize:

@@ -15,7 +15,7 @@
 #define SCAN_ELEMS (2 * SCAN_BLOCKSIZE)
 
 static const char* kKernelNames[FUNC_MAX] = {
-    "insertParticles",   "countingSortFull", "computePressure", "computeForce",
+    "insertParticles",   "countingSortFullRanked", "computePressure", "computeForce",
     "advanceParticles",  "prefixSum",        "prefixFixup",     "countingSortIndex",
     "fxFillUint",        "advanceParticlesClock", "fxClockTick",     "fxCountPairs",
     "insertParticlesBatch", "countingSortIndexBatch", "countingSortFullBatch", "computePressureBatch",

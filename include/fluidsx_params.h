@@ -78,7 +78,8 @@ extern "C" {
  * counted, and countingSortFull ranks every particle by its insert slot
  * (gndx) and moves the reference's eight buffers.  libfluidsx sets both bits. */
 #define FX_CAP_TAIL 1   /* AGRIDCNT/AGRIDOFF hold gridTotal+1 entries (tail bucket) */
-#define FX_CAP_RANKED 2 /* countingSortIndex runs before countingSortFull (stable in-cell order) */
+#define FX_CAP_RANKED 2 /* countingSortIndex runs before countingSortFull (stable in-cell order);
+                         * countingSortFullRanked is that sort alone, the faster kernel */
 
 #define FX_DMAXBUF 16 /* datax.h:37 */
 

@@ -47,6 +47,12 @@ def test_summarize_cuts_the_window_between_the_marks(tmp_path):
                           str(tmp_path / "t.json")], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0, out.stderr
     assert "| computeForce | 2 | 100.0 |" in out.stdout
+    # other counters: mean over the window's dispatches
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--summarize", "--line",
+                          str(tmp_path / "line.json"), "--counters", str(tmp_path / "f.csv")],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert "| computeForce | 2 | 1000 |" in out.stdout
     t = json.loads((tmp_path / "t.json").read_text())
     assert t["window"] == {"steps": 2, "warmup": 5, "autotuned": True, "graph": True}
     assert t["window_avg_us"]["computeForce"] == 100.0

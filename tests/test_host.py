@@ -37,7 +37,8 @@ def test_code_object_exports_the_module_contract():
     data = open(co, "rb").read()
     assert b"gfx950" in data
     for k in ("insertParticles", "countingSortFull", "computePressure", "computeForce", "advanceParticles",
-              "prefixSum", "prefixFixup", "countingSortIndex", "FParams", "FPnts", "FPntTmp", "FAccel"):
+              "prefixSum", "prefixFixup", "countingSortIndex", "FParams", "FPnts", "FPntTmp", "FAccel",
+              "countingSortFullRanked", "FHostCaps", "FPpt"):
         assert b"\x00" + k.encode() + b"\x00" in data, k
 
 
