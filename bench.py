@@ -64,7 +64,8 @@ TRAFFIC_FILES = {"dambreak_4m": "traffic_dambreak_4M_literal.json", "dambreak_4m
 
 BYTES_PER_PARTICLE = {"insert": 20, "prefix": 0, "sort": 92, "pressure": 20, "force": 44, "advance": 88}
 BYTES_PER_CELL = {"insert": 4, "prefix": 8, "sort": 4, "pressure": 0, "force": 0, "advance": 0}
-STAGE_KERNEL = {"insert": "insertParticles", "prefix": "prefixSum", "sort": "countingSortFull",
+STAGES = ("insert", "prefix", "sort", "pressure", "force", "advance")  # the library's stage ids, in order
+STAGE_KERNEL = {"insert": "insertParticles", "prefix": "prefixSum", "sort": "countingSortFullRanked",
                 "pressure": "computePressure", "force": "computeForce", "advance": "advanceParticlesClock"}
 # dambreak_4m as configured: 20 warm-up steps, 200 timed
 CONFIGURED = {"dambreak_4m": (20, 200)}
@@ -201,15 +202,18 @@ def cpu_baseline(p, start, N, workload, args):
     return out
 
 
-def traffic_for(workload, kernel, window):
+def traffic_for(workload, kernel, window, launch):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
-    this workload, only if that summary was taken over the same window."""
+    this workload, only if that summary was taken over the same window and the
+    kernel ran with the same (threads per block, particles per thread)."""
     tf = os.path.join(ROOT, "profiles", TRAFFIC_FILES.get(workload, "-"))
     if not os.path.exists(tf):
         return None, None
     with open(tf) as fh:
         t = json.load(fh)
     if t.get("window") != window or kernel not in t.get("bytes_per_launch", {}):
+        return None, None
+    if list(t.get("launch", {}).get(kernel, [])) != list(launch):
         return None, None
     return round(t["bytes_per_launch"][kernel]), "profiles/" + os.path.basename(tf)
 
@@ -283,7 +287,8 @@ def plain_bench(args, workload, K, W, extras=True, cpu=True):
     stages = stage_table(stage_ms, N, G, C, P)
     roof = roofline(stage_ms, N, G, C, P)
     window = {"steps": K, "warmup": W, "autotuned": tuned is not None, "graph": use_graph}
-    roof["traffic"], src = traffic_for(workload, roof["kernel"], window)
+    si = STAGES.index(next(k for k, v in STAGE_KERNEL.items() if v == roof["kernel"]))
+    roof["traffic"], src = traffic_for(workload, roof["kernel"], window, (thr[si], ps.StagePpt(si)))
     if src:
         roof["traffic_source"] = src
 
@@ -601,7 +606,10 @@ def summarize(args):
     print("\n".join(out))
     if args.json:
         with open(args.json, "w") as fh:
-            json.dump({"workload": line["config"]["workload"].split(":")[0], "window": win,
+            cfg = line["config"]
+            launch = {STAGE_KERNEL[st]: [cfg["stage_threads"][i], cfg.get("stage_ppt", [1] * 6)[i]]
+                      for i, st in enumerate(STAGES)}
+            json.dump({"workload": cfg["workload"].split(":")[0], "window": win, "launch": launch,
                        "bytes_per_launch": traffic,
                        "window_avg_us": {k: float(np.mean(v)) for k, v in dur.items()},
                        "fetch_factor": args.fetch_factor, "write_factor": args.write_factor,

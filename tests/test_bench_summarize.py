@@ -38,8 +38,9 @@ def test_summarize_cuts_the_window_between_the_marks(tmp_path):
     write_csv(tmp_path / "kt.csv", trace, f1)
     write_csv(tmp_path / "f.csv", fetch, f2)
     write_csv(tmp_path / "w.csv", write, f2)
-    line = {"value": 1.0, "ms_per_step": 0.15, "config": {"workload": "dambreak_4m: x", "timed_window": {
-        "steps": 2, "warmup": 5, "autotuned": True, "graph": True}}}
+    line = {"value": 1.0, "ms_per_step": 0.15, "config": {
+        "workload": "dambreak_4m: x", "stage_threads": [64, 128, 1024, 256, 256, 128], "stage_ppt": [1, 1, 2, 1, 1, 1],
+        "timed_window": {"steps": 2, "warmup": 5, "autotuned": True, "graph": True}}}
     (tmp_path / "line.json").write_text("noise\n" + json.dumps(line) + "\n")
     out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--summarize", "--line",
                           str(tmp_path / "line.json"), "--trace", str(tmp_path / "kt.csv"), "--fetch",
@@ -58,3 +59,19 @@ def test_summarize_cuts_the_window_between_the_marks(tmp_path):
     assert t["window_avg_us"]["computeForce"] == 100.0
     # FETCH x2 + WRITE x1, KiB -> bytes
     assert t["bytes_per_launch"]["computeForce"] == (1000.0 * 2 + 10.0) * 1024
+    assert t["launch"]["computeForce"] == [256, 1] and t["launch"]["countingSortFullRanked"] == [1024, 2]
+
+
+def test_traffic_attaches_only_to_the_same_window_and_launch(tmp_path, monkeypatch):
+    sys.path.insert(0, ROOT)
+    import bench
+
+    os.makedirs(tmp_path / "profiles")
+    win = {"steps": 20, "warmup": 5, "autotuned": True, "graph": True}
+    (tmp_path / "profiles" / "traffic_dambreak_4M_literal.json").write_text(json.dumps(
+        {"window": win, "launch": {"computeForce": [256, 1]}, "bytes_per_launch": {"computeForce": 5.0e8}}))
+    monkeypatch.setattr(bench, "ROOT", str(tmp_path))
+    assert bench.traffic_for("dambreak_4m", "computeForce", win, (256, 1))[0] == 500000000
+    assert bench.traffic_for("dambreak_4m", "computeForce", win, (128, 1)) == (None, None)
+    assert bench.traffic_for("dambreak_4m", "computeForce", dict(win, steps=200), (256, 1)) == (None, None)
+    assert bench.traffic_for("dambreak_4m", "computePressure", win, (256, 1)) == (None, None)
