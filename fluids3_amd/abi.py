@@ -12,7 +12,9 @@ GRID_UNDEF = 2147483647  # fluid.h:12
 FPOS, FVEL, FCLR, FVEVAL, FFORCE, FPRESS, FGCELL, FGNDX, FID = range(9)
 # grid buffers (fluid.h:25-31)
 AGRID, AGRIDCNT, AGRIDOFF, AAUXARRAY1, AAUXSCAN1, AAUXARRAY2, AAUXSCAN2 = range(7)
-ANBRBITS, AFREC, ATAILFLAG, ATAILSCAN, ATAILAUX1, ATAILAUXS1, ATAILAUX2, ATAILAUXS2 = range(7, 15)
+ANBRBITS, AFREC, ATAILFLAG, ATAILSCAN, ATAILAUX1, ATAILAUXS1, ATAILAUX2, ATAILAUXS2, ASLABSTART = range(7, 16)
+FX_SLABS = 16
+FX_MEMBER_BITS = 28  # AGRID member entries: index in the low bits, x slab above
 NBR_WORDS = 256  # FX_NBR_WORDS: bitmask words per particle (8192 candidates)
 
 # stages of Particles::Run (particles.cpp:382-394)

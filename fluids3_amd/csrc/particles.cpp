@@ -321,6 +321,9 @@ int Particles::FinishRestart(int n) {
 
 int Particles::ReallocateParticles(int maxpnt) {
   // particles.cpp:267-300, plus the FID slot; GL interop flags become device memory
+  if (maxpnt < 0 || maxpnt >= (1 << FX_MEMBER_BITS))
+    return fx::fail(FX_ERANGE, "ReallocateParticles: %d particles (the ranked sort's member entries hold "
+                               "fewer than 2^%d)", maxpnt, FX_MEMBER_BITS);
   mMaxPoints = maxpnt;
   m_Params.pnum = maxpnt;
   m_Points.DeleteAllBuffers();
@@ -366,6 +369,7 @@ int Particles::RebuildAccelGrid() {
   FX_TRY(m_Accel.AddBuffer(AAUXSCAN2, "scan2", sizeof(unsigned), numElem3, DT_CUMEM));
   FX_TRY(m_Accel.AddBuffer(ANBRBITS, "nbrbits", sizeof(unsigned), (uint64_t)FX_NBR_WORDS * mMaxPoints, DT_CUMEM));
   FX_TRY(m_Accel.AddBuffer(AFREC, "frec", 32, mMaxPoints, DT_CUMEM));
+  FX_TRY(m_Accel.AddBuffer(ASLABSTART, "slabstart", sizeof(unsigned), (uint64_t)FX_SLABS * numElem1, DT_CUMEM));
   // out-of-range flags of the particles and their scan levels
   const int tail1 = mMaxPoints > 0 ? mMaxPoints : 1;
   const int tail2 = tail1 / SCAN_ELEMS + 1;

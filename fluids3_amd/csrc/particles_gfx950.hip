@@ -381,37 +381,62 @@ extern "C" __global__ __launch_bounds__(SCAN_BLOCKSIZE) void prefixFixup(
 
 // ------------------------------------------------------------ counting sort
 
-// Index sort: AGRID[off[cell] + gndx] = i, the member list of every cell in
-// insertion (atomic) order.  Same role as fluids3.2's countingSortIndex
-// (fluids3.2/fluids/fluid_system_kern.cu:139-150); reads the pre-sort set
-// FPntTmp.  Out-of-range particles fill their scene's tail bucket.
+// x slab of an in-range position inside its cell, from the insert's float
+// cell coordinate gx: floor(frac(gx) * FX_SLABS) (gx - (int)gx is exact for
+// gx >= 1, and the product by a power of two is exact).
+__device__ __forceinline__ int slab_of(const FParams_t& P, float px) {
+  const float gx = (px - P.gridMin.x) * P.gridDelta.x;
+  const float f = gx - (float)(int)gx;
+  return min((int)(f * (float)FX_SLABS), FX_SLABS - 1);
+}
+
+// Index sort: AGRID[off[cell] + gndx] = i | slab << FX_MEMBER_BITS, the
+// member list of every cell in insertion (atomic) order, each entry carrying
+// the member's x slab for the full sort.  Same role as fluids3.2's
+// countingSortIndex (fluids3.2/fluids/fluid_system_kern.cu:139-150); reads
+// the pre-sort set FPntTmp.  Out-of-range particles fill their scene's tail
+// bucket (slab 0).
+#define FX_MEMBER_MASK ((1u << FX_MEMBER_BITS) - 1u)
 __device__ __forceinline__ void sort_index_impl(const FParams_t& P, const Span S) {
   const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
   if (i >= S.p1) return;
   const int c = pbuf<const int>(FPntTmp, FGCELL)[i];
   const int cell = (c == FX_GRID_UNDEF) ? S.c0 + P.gridTotal : c;
   const uint* off = pbuf<const uint>(FAccel, AGRIDOFF);
-  pbuf<uint>(FAccel, AGRID)[off[cell] + pbuf<const int>(FPntTmp, FGNDX)[i]] = (uint)i;
+  const uint slot = off[cell] + (uint)pbuf<const int>(FPntTmp, FGNDX)[i];
+  const uint slab = c == FX_GRID_UNDEF ? 0u : (uint)slab_of(P, pbuf<const fx_float3>(FPntTmp, FPOS)[i].x);
+  pbuf<uint>(FAccel, AGRID)[slot] = (uint)i | (slab << FX_MEMBER_BITS);
 }
 
-// Full counting sort FPntTmp -> FPnts (particles.cu:64-95) with a stable
-// in-cell order: a particle's slot in its cell is the number of members with
-// a smaller pre-sort index (read from the member list), so the sorted order
-// equals a stable sort by cell key and does not depend on atomic timing.
-// The tail bucket of out-of-range particles, which can hold many, is ranked
-// by the exclusive scan of the insert stage's out-of-range flags instead.
-// Cells above FX_RANK_CAP members keep the atomic order.  The sort moves the
-// state that outlives a step (pos, vel, veval, clr, id) and the fresh gcell
-// and gndx; press and force are recomputed by the next two stages.
+// Full counting sort FPntTmp -> FPnts (particles.cu:64-95) with a fixed
+// in-cell order: a particle's slot in its cell is the number of members that
+// precede it by (x slab, pre-sort index) -- the cell split into FX_SLABS
+// equal slabs along x (slab_of) -- read from the member list, so the sorted
+// order does not depend on atomic timing, and every cell's members lie in
+// ascending slabs, which lets the density stage trim each stencil row to the
+// particle's x-range.  The tail bucket of out-of-range particles, which can
+// hold many, is ranked by the exclusive scan of the insert stage's
+// out-of-range flags instead (particle order).  Cells above FX_RANK_CAP
+// members keep the atomic order.  The sort moves the state that outlives a
+// step (pos, vel, veval, clr, id) and the fresh gcell and gndx; press and
+// force are recomputed by the next two stages.
 //
-// Cells of more than FX_RANK_WAVE members are ranked by the wave together:
-// the particles of a wave are consecutive, so the members of a cell below
-// lane l's index are the members below the wave's first index w0 (counted
-// once per distinct cell, 64 members per load) plus the wave's lanes below l
-// in the same cell.  Smaller cells are ranked per lane.
+// A member entry is (slab << FX_MEMBER_BITS) | index, so "precedes" is one
+// unsigned comparison of entries.  Cells of up to FX_RANK_WAVE members are
+// ranked per lane that way.  Larger ones are ranked by the wave together, one
+// distinct cell at a time: per 64 member entries the wave takes five ballots
+// (valid, and the four slab bits) and every lane walks the slab bits from the
+// top, keeping the members that agree with its slab so far and counting
+// those that fall below it; the members in its own slab with an index below
+// the wave's first, and the wave's own lanes below it in the same cell and
+// slab, complete the rank.  The same walk with slab = lane gives where each
+// slab begins, which lanes 0..FX_SLABS-1 store in ASLABSTART (when the host
+// has it) for the density stage; smaller cells get no slab starts.
 #ifndef FX_RANK_WAVE
-#define FX_RANK_WAVE 32
+#define FX_RANK_WAVE 16
 #endif
+#define FX_SLAB_BITS 4
+static_assert((1 << FX_SLAB_BITS) == FX_SLABS, "FX_SLABS is a power of two");
 
 __device__ __forceinline__ void sort_full_impl(const FParams_t& P, const Span S) {
   const int lane = (int)(threadIdx.x & 63u);
@@ -421,29 +446,53 @@ __device__ __forceinline__ void sort_full_impl(const FParams_t& P, const Span S)
   const uint* cnt = pbuf<const uint>(FAccel, AGRIDCNT);
   const uint* grid = pbuf<const uint>(FAccel, AGRID);
   const uint* tscan = pbuf<const uint>(FAccel, ATAILSCAN);
+  uint* sstart = pbuf<uint>(FAccel, ASLABSTART);
   const int c = act ? pbuf<const int>(FPntTmp, FGCELL)[i] : FX_GRID_UNDEF;
   const int cell = (c == FX_GRID_UNDEF) ? S.c0 + P.gridTotal : c;
   const uint o = act ? off[cell] : 0u;
   const uint n = act ? cnt[cell] : 0u;
+  const fx_float3 pos = act ? pbuf<const fx_float3>(FPntTmp, FPOS)[i] : fx_float3{0.0f, 0.0f, 0.0f};
+  const int slab = c == FX_GRID_UNDEF ? 0 : slab_of(P, pos.x);
+  const uint key = ((uint)slab << FX_MEMBER_BITS) | (uint)i;  // this particle's own member entry
   const bool wide = act && c != FX_GRID_UNDEF && n > FX_RANK_WAVE && n <= FX_RANK_CAP;
   int rank = 0;
   {
     const uint w0 = (uint)(i - lane);
+    const int tq = lane & (FX_SLABS - 1);  // the slab whose start this lane counts
+    const unsigned long long lower = (1ull << lane) - 1ull;
     unsigned long long pending = __ballot(wide);
     while (pending != 0ull) {  // one distinct dense cell per pass (wave-uniform)
       const int lead = __ffsll((long long)pending) - 1;
       const int cl = __shfl(cell, lead, 64);
       const uint ol = (uint)__shfl((int)o, lead, 64), nl = (uint)__shfl((int)n, lead, 64);
-      int below = 0;
+      int below = 0, early = 0, start = 0;
       for (uint k0 = 0; k0 < nl; k0 += 64) {
         const uint k = k0 + (uint)lane;
-        const uint m = k < nl ? grid[ol + k] : 0xffffffffu;
-        below += __popcll(__ballot(m < w0));
+        const bool valid = k < nl;
+        const uint e = valid ? grid[ol + k] : 0u;
+        unsigned long long m1 = __ballot(valid), m2 = m1;
+        const unsigned long long low = __ballot(valid && (e & FX_MEMBER_MASK) < w0);
+#pragma unroll
+        for (int b = FX_SLAB_BITS - 1; b >= 0; b--) {
+          const unsigned long long bit = __ballot(valid && ((e >> (FX_MEMBER_BITS + b)) & 1u));
+          const bool s1 = (slab >> b) & 1, s2 = (tq >> b) & 1;
+          below += s1 ? __popcll(m1 & ~bit) : 0;
+          m1 = s1 ? (m1 & bit) : (m1 & ~bit);
+          start += s2 ? __popcll(m2 & ~bit) : 0;
+          m2 = s2 ? (m2 & bit) : (m2 & ~bit);
+        }
+        early += __popcll(m1 & low);
       }
       const bool mine = wide && cell == cl;
-      const unsigned long long same = __ballot(mine);
-      if (mine) rank = below + __popcll(same & ((1ull << lane) - 1ull));
-      pending &= ~same;
+      unsigned long long m3 = __ballot(mine);
+#pragma unroll
+      for (int b = FX_SLAB_BITS - 1; b >= 0; b--) {
+        const unsigned long long bit = __ballot(mine && ((slab >> b) & 1));
+        m3 = ((slab >> b) & 1) ? (m3 & bit) : (m3 & ~bit);
+      }
+      if (mine) rank = below + early + __popcll(m3 & lower);
+      if (sstart != nullptr && lane < FX_SLABS) sstart[(size_t)cl * FX_SLABS + lane] = (uint)start;
+      pending &= ~__ballot(mine);
     }
   }
   if (!act) return;
@@ -453,24 +502,22 @@ __device__ __forceinline__ void sort_full_impl(const FParams_t& P, const Span S)
     // tail bucket: the number of the scene's out-of-range particles before i
     rank = (int)(tscan[i] - tscan[S.p0]);
   } else if (n <= FX_RANK_CAP) {
-    // members with a smaller pre-sort index; eight loads in flight per step
-    rank = 0;
+    // member entries below this particle's own; eight in flight per step
     uint k = 0;
     for (; k + 8 <= n; k += 8) {
-      uint g[8];
+      uint e[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) g[u] = grid[o + k + u];
+      for (int u = 0; u < 8; u++) e[u] = grid[o + k + u];
 #pragma unroll
-      for (int u = 0; u < 8; u++) rank += (g[u] < (uint)i) ? 1 : 0;
+      for (int u = 0; u < 8; u++) rank += e[u] < key ? 1 : 0;
     }
-    for (; k < n; k++) rank += (grid[o + k] < (uint)i) ? 1 : 0;
+    for (; k < n; k++) rank += grid[o + k] < key ? 1 : 0;
   } else {
     rank = pbuf<const int>(FPntTmp, FGNDX)[i];
   }
   // every load before the first store (the two sets are distinct buffers,
   // which the compiler cannot know)
   const bool has_id = FPntTmp.mbuf[FID] != 0 && FPnts.mbuf[FID] != 0;
-  const fx_float3 pos = pbuf<const fx_float3>(FPntTmp, FPOS)[i];
   const fx_float3 vel = pbuf<const fx_float3>(FPntTmp, FVEL)[i];
   const fx_float3 vev = pbuf<const fx_float3>(FPntTmp, FVEVAL)[i];
   const uint clr = pbuf<const uint>(FPntTmp, FCLR)[i];
@@ -661,7 +708,40 @@ __device__ __forceinline__ int nbr_words(const FParams_t& P, int gc, const uint*
 // at [w * N + i], N the size of the whole particle buffer) and the particle's
 // (pos, rho), (veval, 0) record.  A particle whose candidates need more than
 // FX_NBR_WORDS words records no bits.
-__device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, int N, float4* tile, bool tail) {
+//
+// Row trimming (when the sort ordered every cell's members by x slab and
+// wrote where each slab begins, `trim`).  In grid units (the insert's
+// gx = (x - gridMin.x) * gridDelta.x, cells of width 1) a member of slab k of
+// a cell at x index c has gx in [c + k/16, c + (k+1)/16).  A particle at gp
+// is out of radius of a whole slab of its row's first cell (x index cx - 1)
+// when gp - (cx - 1) - (k+1)/16 > Rg, and of a whole slab of its row's last
+// cell (x index cx + srch - 2) when (cx + srch - 2) + k/16 - gp > Rg, with
+// Rg = rlim * gridDelta.x plus a margin of 2^-7 cells and gridRes.x * 2^-20
+// that covers the rounding of gx (about 2^-22 of it) on both sides: every
+// member of such a slab has fl(|px - x|) > rlim, so fl(dx * dx) > rd2 and it
+// fails the radius test whatever its y and z.  Those slabs are a prefix of
+// the first cell and a suffix of the last (the members lie in ascending
+// slabs), located through the slab starts, and the lane walks only [ts, te)
+// of its row.  The skipped candidates' bits are zero: the bitmask keeps its
+// layout (bit s of the row for candidate s), written with the lane's phase
+// ts % 32.
+#define FX_TRIM_MARGIN 0.0078125f
+
+// Slabs [0, lo) of the row's first cell and [hi, FX_SLABS) of its last cell
+// lie beyond the radius of a particle at grid x coordinate gp in cell cx.
+__device__ __forceinline__ void trim_slabs(float gp, int cx, int srch, float Rg, int& lo, int& hi) {
+  const float fx = (float)(cx - 1), lx = (float)(cx + srch - 2);
+  lo = min(max((int)ceilf((float)FX_SLABS * (gp - fx - Rg) - 1.0f), 0), FX_SLABS);
+  hi = min(max((int)floorf((float)FX_SLABS * (gp - lx + Rg)) + 1, 0), FX_SLABS);
+}
+
+// Members of cell c (n of them) in slabs below k.
+__device__ __forceinline__ int slab_start(const uint* __restrict__ sstart, int c, int n, int k) {
+  return (n == 0 || k == 0) ? 0 : (k >= FX_SLABS ? n : (int)sstart[(size_t)c * FX_SLABS + k]);
+}
+
+__device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, int N, float4* tile, bool tail,
+                                              bool trim) {
   const int t = threadIdx.x;
   const int h0 = S.p0 + S.blk * (int)blockDim.x;
   const int i = h0 + t;
@@ -682,6 +762,14 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
   const float rd2 = P.rd2, d2 = P.d2;
   const size_t NN = (size_t)N;
   const bool rec = wbits && act && nbr_words(P, gc, off, cnt) <= nbr_cap();
+  // x-distance beyond which fl(dx * dx) > rd2: dx > rlim > sqrt(rd2) (1 + 2^-21)
+  const float rlim = __fsqrt_rn(rd2) * (1.0f + 0x1p-20f);
+  const float Rg = rlim * P.gridDelta.x + FX_TRIM_MARGIN + (float)P.gridRes.x * 0x1p-20f;
+  const float gp = (p.x - P.gridMin.x) * P.gridDelta.x;  // the insert's cell coordinate
+  const uint* __restrict__ sstart = pbuf<const uint>(FAccel, ASLABSTART);
+  const bool trim_lane = trim && sstart != nullptr && act;
+  int slo = 0, shi = FX_SLABS;
+  if (trim_lane) trim_slabs(gp, (int)gp, srch, Rg, slo, shi);
   float sum = 0.0f;
   int W = 0;  // first word of the current row
   for (int r = 0; r < srch * srch; r++) {
@@ -692,6 +780,15 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
     if (act) {
       jb = (int)off[gc + adj];
       je = (int)(off[gc + adj + srch - 1] + cnt[gc + adj + srch - 1]);
+    }
+    // the lane walks candidates [ts, te) of its row (relative to jb)
+    int ts = 0, te = je - jb;
+    if (trim_lane) {
+      const int c0 = gc + adj, c2 = gc + adj + srch - 1;
+      const int n0 = (int)cnt[c0], n2 = (int)cnt[c2];
+      // slab starts exist for the cells the sort ranked by wave
+      if (n0 > FX_RANK_WAVE && n0 <= FX_RANK_CAP) ts = slab_start(sstart, c0, n0, slo);
+      if (n2 > FX_RANK_WAVE && n2 <= FX_RANK_CAP) te = max(ts, te - n2 + slab_start(sstart, c2, n2, shi));
     }
     if (hi - lo <= TILE_P) {
       // The row's run fits one tile: the lane's candidate s (0 <= s < len)
@@ -706,18 +803,19 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
       }
       __syncthreads();
       const int a = act ? jb - lo : 0, len = act ? je - jb : 0;
-      const int lmax = wave_max(len), lmin = wave_min(act ? len : INT_MAX);
+      const int L = te - ts, a0 = a + ts, ph = ts & 31, w0 = ts >> 5;
+      const int lmax = wave_max(L), lmin = wave_min(act ? L : INT_MAX);
       const int last = hi - lo - 1;  // last entry of the run
       const int nw = (len + 31) >> 5;
       uint acc = 0u;
       auto cand = [&](int s, auto checked) {
-        int k = a + s;
+        int k = a0 + s;
         if (decltype(checked)::value) k = min(k, last);
         const float4 q = tile[k];
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         if (decltype(checked)::value) {
-          const bool pair = dsq < rd2 && dsq > 0.0f && s < len;
+          const bool pair = dsq < rd2 && dsq > 0.0f && s < L;
           const float tt = pair ? (rd2 - dsq) * d2 : 0.0f;
           sum = fmaf(tt * tt, tt, sum);  // + 0 when not a pair: sum unchanged
           acc = acc + acc + (pair ? 1u : 0u);
@@ -738,6 +836,11 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
           acc = acc2;
         }
       };
+      // the block of candidates ts + s0 .. ts + s0 + 31 straddles words
+      // w0 + s0/32 (bits ph..31) and the next (bits 0..ph-1): the second
+      // part is carried into the next block's word
+      uint carry = 0u;
+      int wb = w0;
       for (int s0 = 0; s0 < lmax; s0 += 32) {
         const int n = min(32, lmax - s0);
         const int nu = max(0, min(n, lmin - s0));
@@ -751,15 +854,26 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
         }
         for (; u < nu; u++) cand(s0 + u, std::false_type());
         for (; u < n; u++) cand(s0 + u, std::true_type());
-        if (rec && (s0 >> 5) < nw) bits[(size_t)(W + (s0 >> 5)) * NN + i] = __builtin_bitreverse32(acc << (32 - n));
+        const uint chunk = __builtin_bitreverse32(acc << (32 - n));
+        if (rec && wb < nw) bits[(size_t)(W + wb) * NN + i] = carry | (chunk << ph);
+        carry = ph ? chunk >> (32 - ph) : 0u;
+        wb++;
         acc = 0u;
+      }
+      if (rec) {  // the words the walk did not reach hold no pair
+        for (int w = 0; w < w0; w++) bits[(size_t)(W + w) * NN + i] = 0u;
+        if (wb < nw) bits[(size_t)(W + wb) * NN + i] = carry;
+        for (int w = wb + 1; w < nw; w++) bits[(size_t)(W + w) * NN + i] = 0u;
       }
       __syncthreads();
       W += (je - jb + 31) >> 5;
       continue;
     }
-    uint acc = 0, bit = 1u;  // bits of the current word, and the next one's mask
-    int wcur = W;            // current word
+    // bits of the current word and the next one's mask, from the lane's phase
+    uint acc = 0, bit = 1u << (ts & 31);
+    int wcur = W + (ts >> 5);  // current word
+    if (rec)
+      for (int w = W; w < wcur; w++) bits[(size_t)w * NN + i] = 0u;  // skipped candidates
     for (int c0 = lo; c0 < hi; c0 += TILE_P) {
       const int n = min(TILE_P, hi - c0);
       for (int k = t; k < n; k += blockDim.x) {
@@ -767,7 +881,7 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
         tile[k] = make_float4(q.x, q.y, q.z, 0.0f);
       }
       __syncthreads();
-      const int a = max(jb, c0) - c0, b = min(je, c0 + n) - c0;
+      const int a = max(jb + ts, c0) - c0, b = min(jb + te, c0 + n) - c0;
       for (int k = a; k < b; k++) {
         const float4 q = tile[k];
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
@@ -789,7 +903,9 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
       }
       __syncthreads();
     }
-    if (rec && bit != 1u) bits[(size_t)wcur * NN + i] = acc;  // the row's partial last word
+    if (rec && bit != 1u) bits[(size_t)wcur++ * NN + i] = acc;  // the walk's partial last word
+    if (rec)
+      for (int w = wcur; w < W + ((je - jb + 31) >> 5); w++) bits[(size_t)w * NN + i] = 0u;
     W += (je - jb + 31) >> 5;
   }
   if (act) {
@@ -807,14 +923,14 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
 
 extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
   __shared__ float4 tile[TILE_P];
-  pressure_impl(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, tile, host_tail());
+  pressure_impl(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, tile, host_tail(), host_ranked());
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computePressureBatch(BatchArgs B, int ntotal) {
   __shared__ float4 tile[TILE_P];
   const FParams_t* P;
   const Span S = batch_span(B, xcd_block(), P);
-  pressure_impl(*P, S, ntotal, tile, true);
+  pressure_impl(*P, S, ntotal, tile, true, true);
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computePressureBatchP(BatchArgs B, int ntotal) {
@@ -822,7 +938,7 @@ extern "C" __global__ __launch_bounds__(1024) void computePressureBatchP(BatchAr
   batch_items(B, [&](int b) {
     const FParams_t* P;
     const Span S = batch_span(B, b, P);
-    pressure_impl(*P, S, ntotal, tile, true);
+    pressure_impl(*P, S, ntotal, tile, true, true);
     __syncthreads();  // the next entry's tile loads wait for this entry's last reads
   });
 }

@@ -18,6 +18,9 @@
  *     results can be compared per particle.
  *   - ANBRBITS, AFREC: neighbour bitmask and packed per-particle record that
  *     the density stage leaves for the force stage.
+ *   - ASLABSTART: where each x slab of a cell begins, written by the full
+ *     sort for the density stage, which skips the slabs of a stencil row that
+ *     lie beyond a particle's radius.
  *   - ATAILFLAG .. ATAILAUXS2: out-of-range flags and their exclusive scan,
  *     which give the tail bucket a stable (particle-order) ranking however
  *     many particles it holds.
@@ -68,8 +71,11 @@ extern "C" {
 #define ATAILAUXS1 12
 #define ATAILAUX2 13
 #define ATAILAUXS2 14
-#define FX_NUM_ABUF 15
+#define ASLABSTART 15 /* per cell c and x slab k: members of c in slabs below k, uint32 at [c * FX_SLABS + k] */
+#define FX_NUM_ABUF 16
 #define FX_NBR_WORDS 256 /* bitmask words per particle (8192 candidates) */
+#define FX_SLABS 16      /* x slabs per cell: the in-cell order of the ranked sort */
+#define FX_MEMBER_BITS 28 /* AGRID member entries: index in the low 28 bits, x slab above */
 
 /* Module constant FHostCaps (int, default 0): what the host allocated and
  * runs beyond the reference's layout.  A host that binds the code object the
@@ -78,8 +84,10 @@ extern "C" {
  * counted, and countingSortFull ranks every particle by its insert slot
  * (gndx) and moves the reference's eight buffers.  libfluidsx sets both bits. */
 #define FX_CAP_TAIL 1   /* AGRIDCNT/AGRIDOFF hold gridTotal+1 entries (tail bucket) */
-#define FX_CAP_RANKED 2 /* countingSortIndex runs before countingSortFull (stable in-cell order);
-                         * countingSortFullRanked is that sort alone, the faster kernel */
+#define FX_CAP_RANKED 2 /* countingSortIndex runs before countingSortFull (fixed in-cell order: x slab,
+                         * then particle index); countingSortFullRanked is that sort alone, the faster
+                         * kernel.  AGRID then holds member entries (index | slab << FX_MEMBER_BITS), so
+                         * the host keeps fewer than 2^FX_MEMBER_BITS particles */
 
 #define FX_DMAXBUF 16 /* datax.h:37 */
 

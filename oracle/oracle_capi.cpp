@@ -80,7 +80,7 @@ void run(const FParams_t* P, oracle::State<R>& S, int nsteps, int stage_mask, in
   for (int s = 0; s < nsteps; s++) {
     if (stage_mask & 1) oracle::insert(*P, S);
     if (stage_mask & 2) oracle::scan(S);
-    if (stage_mask & 4) oracle::sort(S);
+    if (stage_mask & 4) oracle::sort(*P, S, threads);
     if (stage_mask & 8) oracle::pressure(*P, S, threads);
     if (stage_mask & 16) oracle::force(*P, S, threads);
     if (stage_mask & 32) {

@@ -73,7 +73,7 @@ void rollout(const FParams_t& P, State<R>& S, int nsteps, std::vector<TapeStep<R
   for (int t = 0; t < nsteps; t++) {
     insert(P, S);
     scan(S);
-    sort(S);
+    sort(P, S, threads);
     pressure(P, S, threads);
     force(P, S, threads);
     if (tape) record(S, (*tape)[t]);

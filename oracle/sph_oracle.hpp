@@ -21,6 +21,7 @@
 //     overwrite them for every in-range particle).
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -108,23 +109,59 @@ void scan(State<R>& S) {
 
 // ----------------------------------------------------------------- sort
 // fluids3.9 fluid_system.cpp:496-517 / particles.cu:64-95: scatter to
-// off[cell] + gndx; grid[] receives the pre-sort index (index sort,
-// fluids3.2/fluids/fluid_system_kern.cu:139-150).
+// off[cell] + rank; grid[] receives the pre-sort index (index sort,
+// fluids3.2/fluids/fluid_system_kern.cu:139-150).  The reference's in-cell
+// order is the insert's atomic order (particles.cu:41), which no run
+// reproduces; this step fixes it: the members of a cell are ordered by their
+// x slab (the cell split into FX_SLABS = 16 equal slabs along x, slab_of below),
+// ties by pre-sort index, and gndx becomes that rank.  Cells of more than
+// FX_RANK_CAP members and the out-of-range tail keep particle order.
+#ifndef FX_RANK_CAP
+#define FX_RANK_CAP 65536
+#endif
+
+// x slab of an in-range position inside its cell, from the insert's float
+// cell coordinate gx: floor(frac(gx) * FX_SLABS) (gx - (int)gx is exact for
+// gx >= 1, and the product by a power of two is exact).
+inline int slab_of(const FParams_t& P, float px) {
+  const float gx = (px - P.gridMin.x) * P.gridDelta.x;
+  const float f = gx - (float)(int)gx;
+  const int s = (int)(f * (float)FX_SLABS);
+  return s < FX_SLABS - 1 ? s : FX_SLABS - 1;
+}
+
 template <class R>
-void sort(State<R>& S) {
+void sort(const FParams_t& P, State<R>& S, int threads = 1) {
   const State<R> T = S;  // the reference's CopyAllToTemp
+  // member lists in particle order (the sequential insert's gndx)
+  std::vector<uint32_t> mem(S.n);
   for (int i = 0; i < S.n; i++) {
     const int c = T.gcell[i];
     const int cell = (c == FX_GRID_UNDEF) ? S.G : c;
-    const uint32_t s = S.off[cell] + (uint32_t)T.gndx[i];
-    S.pos[s] = T.pos[i];
-    S.vel[s] = T.vel[i];
-    S.veval[s] = T.veval[i];
-    S.clr[s] = T.clr[i];
-    S.id[s] = T.id[i];
-    S.gcell[s] = c;
-    S.gndx[s] = T.gndx[i];
-    S.grid[s] = (uint32_t)i;
+    mem[S.off[cell] + (uint32_t)T.gndx[i]] = (uint32_t)i;
+  }
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(threads) if (threads > 1)
+  for (int c = 0; c < S.G; c++) {
+    const uint32_t n = S.cnt[c];
+    if (n < 2 || n > FX_RANK_CAP) continue;
+    uint32_t* m = mem.data() + S.off[c];
+    std::sort(m, m + n, [&](uint32_t a, uint32_t b) {
+      const int sa = slab_of(P, (float)T.pos[a].x), sb = slab_of(P, (float)T.pos[b].x);
+      return sa != sb ? sa < sb : a < b;
+    });
+  }
+  for (int c = 0; c <= S.G; c++) {
+    for (uint32_t r = 0; r < S.cnt[c]; r++) {
+      const uint32_t s = S.off[c] + r, i = mem[s];
+      S.pos[s] = T.pos[i];
+      S.vel[s] = T.vel[i];
+      S.veval[s] = T.veval[i];
+      S.clr[s] = T.clr[i];
+      S.id[s] = T.id[i];
+      S.gcell[s] = T.gcell[i];
+      S.gndx[s] = (int32_t)r;
+      S.grid[s] = i;
+    }
   }
 }
 
@@ -327,7 +364,7 @@ template <class R>
 void step(const FParams_t& P, State<R>& S, int threads) {
   insert(P, S);
   scan(S);
-  sort(S);
+  sort(P, S, threads);
   pressure(P, S, threads);
   force(P, S, threads);
   advance(P, S, threads);

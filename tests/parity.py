@@ -38,7 +38,12 @@ def assert_int_equal(name, a, b):
 
 
 def cell_members_equal(grid_a, grid_b, off, cnt):
-    """The member list of every cell holds the same particle indices (order free)."""
+    """The member list of every cell holds the same particle indices (order free).
+    Entries are compared without the x slab the ranked sort packs above the
+    index (bits 28 and up)."""
+    mask = np.uint32((1 << 28) - 1)
+    grid_a = np.asarray(grid_a, np.uint32) & mask
+    grid_b = np.asarray(grid_b, np.uint32) & mask
     n = grid_a.shape[0]
     seg = np.repeat(np.arange(cnt.shape[0]), cnt.astype(np.int64))
     assert seg.shape[0] == n, "counts do not cover the list"

@@ -21,8 +21,9 @@ these compressed lattices, so their FP32 value depends on the summation order
 particles in another order, moves by 3e-4 to 9e-4 in force.  Their bar is
 therefore accuracy: against the FP64 build of the same formulas they must be
 no further than 1.5 times the oracle's own distance from it (measured ratios
-under reordered oracle runs: 0.7 to 1.25).  Under the stable in-cell order of
-libfluidsx the same kernels are bit-identical to the oracle (test_gpu_step.py).
+under reordered oracle runs: 0.7 to 1.25).  Under the fixed in-cell order of
+libfluidsx (by x, then particle index) the same kernels are bit-identical to
+the oracle (test_gpu_step.py).
 """
 import os
 import subprocess

@@ -377,10 +377,10 @@ def test_large_tail_bucket_is_stable(ps):
         assert bits_equal(g[k], again[k]), k
 
 
-def test_crowded_cell_is_stable(ps):
+def test_crowded_cell_is_ordered(ps):
     """3,000 particles packed into one cell (more than the 2,048 members a cell
     of the per-lane and earlier ranking covered): the sort ranks the cell
-    wave-cooperatively, stable by particle index, and two steps equal the
+    wave-cooperatively, by x and then particle index, and two steps equal the
     oracle bit for bit."""
     sc = scenes.dambreak_16k()
     ps.Restart(sc)

@@ -142,7 +142,15 @@ def test_scan_is_exclusive_cumsum():
     assert np.array_equal(st["off"], want.astype(np.uint32))
 
 
-def test_sort_is_stable_sort_by_cell():
+def slab_of(P, x):
+    """The x slab of an in-range position in its cell, in float32 like the
+    insert's cell coordinate: floor(frac(gx) * 16)."""
+    gx = (np.asarray(x, F32) - F32(P.gridMin.x)) * F32(P.gridDelta.x)
+    f = gx - gx.astype(np.int32).astype(F32)
+    return np.minimum((f * F32(16)).astype(np.int64), 15)
+
+
+def test_sort_is_by_cell_then_x_slab_then_index():
     sc = scenes.dambreak_16k()
     P = params_for(sc)
     pos, clr = fill_points(sc)
@@ -155,7 +163,10 @@ def test_sort_is_stable_sort_by_cell():
     st = oracle.run(P, oracle.convert(s0, np.float32), 1,
                     oracle.STAGE_INSERT | oracle.STAGE_SCAN | oracle.STAGE_SORT)
     key = np.where(cell_of(pos, P) == GRID_UNDEF, P.gridTotal, cell_of(pos, P))
-    order = np.argsort(key, kind="stable")
+    # in-range cells by x slab, ties by index; the out-of-range tail by index
+    xk = np.where(key == P.gridTotal, 0, slab_of(P, pos[:, 0]))
+    order = np.lexsort((np.arange(pos.shape[0]), xk, key))
+    assert not np.array_equal(order, np.argsort(key, kind="stable"))  # slab order differs from index order
     assert np.array_equal(st["id"], order.astype(np.uint32))
     assert np.array_equal(st["grid"], order.astype(np.uint32))
     assert np.array_equal(st["pos"], pos[order]) and np.array_equal(st["vel"], vel[order])
@@ -165,6 +176,24 @@ def test_sort_is_stable_sort_by_cell():
     # gndx = rank inside the cell, off[cell] + gndx = slot
     slot = st["off"][k].astype(np.int64) + st["gndx"]
     assert np.array_equal(slot, np.arange(pos.shape[0]))
+
+
+def test_sort_in_cell_order_hand_placed():
+    """Five particles of one cell at x = 0.6, 0.3, 0.6, 0.45, 0.27 of the cell
+    edge (slabs 9, 4, 9, 7, 4 of sixteen) and one out of range: the cell holds
+    them by slab, equal slabs by particle index (so 0.3 before 0.27); the
+    tail keeps particle order."""
+    sc = small_box_scene()
+    P = params_for(sc)
+    lo = np.array(P.gridMin.tolist(), np.float64)
+    w = 1.0 / np.array(P.gridDelta.tolist(), np.float64)
+    at = lambda f: (lo + np.asarray(f, np.float64) * w).astype(F32)
+    fx = [5.6, 5.3, 5.6, 5.45, 5.27]
+    pts = [at((f, 6.5 + 0.05 * k, 7.5)) for k, f in enumerate(fx)] + [at((0.5, 5.5, 5.5))]
+    st = oracle.run(P, oracle.make_state(P, np.array(pts, F32)), 1,
+                    oracle.STAGE_INSERT | oracle.STAGE_SCAN | oracle.STAGE_SORT)
+    assert st["id"].tolist() == [1, 4, 3, 0, 2, 5]
+    assert st["gndx"].tolist() == [0, 1, 2, 3, 4, 0]
 
 
 # --------------------------------------------------------------- density
