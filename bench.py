@@ -67,6 +67,9 @@ BYTES_PER_CELL = {"insert": 4, "prefix": 8, "sort": 4, "pressure": 0, "force": 0
 STAGES = ("insert", "prefix", "sort", "pressure", "force", "advance")  # the library's stage ids, in order
 STAGE_KERNEL = {"insert": "insertParticles", "prefix": "prefixSum", "sort": "countingSortFullRanked",
                 "pressure": "computePressure", "force": "computeForce", "advance": "advanceParticlesClock"}
+# the same stages of a batch of scenes (one launch per stage over every scene)
+BATCH_KERNEL = {"insert": "insertParticlesBatch", "prefix": "prefixSum", "sort": "countingSortFullBatch",
+                "pressure": "computePressureBatch", "force": "computeForceBatch", "advance": "advanceParticlesBatch"}
 # dambreak_4m as configured: 20 warm-up steps, 200 timed
 CONFIGURED = {"dambreak_4m": (20, 200)}
 
@@ -100,7 +103,7 @@ def stage_table(stage_ms, N, G, C, P):
     return stages
 
 
-def roofline(stage_ms, N, G, C, P):
+def roofline(stage_ms, N, G, C, P, batch=False):
     """Roof of the dominant stage under the cost model: FP32 vector issue or HBM."""
     dom = max(stage_ms, key=stage_ms.get)
     fl = flops_per_particle(dom, C, P) * N
@@ -114,7 +117,7 @@ def roofline(stage_ms, N, G, C, P):
         ach = by / t_dom / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4)}
-    roof["kernel"] = STAGE_KERNEL[dom]
+    roof["kernel"] = (BATCH_KERNEL if batch else STAGE_KERNEL)[dom]
     roof["kernel_ms"] = round(stage_ms[dom], 5)
     roof["algorithmic_bytes"] = by
     roof["algorithmic_flops"] = fl
@@ -204,14 +207,18 @@ def cpu_baseline(p, start, N, workload, args):
 
 def traffic_for(workload, kernel, window, launch):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary of
-    this workload, only if that summary was taken over the same window and the
-    kernel ran with the same (threads per block, particles per thread)."""
+    this workload, only if that summary was taken over the same window (timed
+    steps, warm-up steps, graph or not) and the kernel ran with the same
+    (threads per block, particles per thread).  Whether the block sizes came
+    from the tuner or were given does not matter: the counter runs pin them."""
     tf = os.path.join(ROOT, "profiles", TRAFFIC_FILES.get(workload, "-"))
     if not os.path.exists(tf):
         return None, None
     with open(tf) as fh:
         t = json.load(fh)
-    if t.get("window") != window or kernel not in t.get("bytes_per_launch", {}):
+    keys = ("steps", "warmup", "graph")
+    tw = t.get("window") or {}
+    if any(tw.get(k) != window.get(k) for k in keys) or kernel not in t.get("bytes_per_launch", {}):
         return None, None
     if list(t.get("launch", {}).get(kernel, [])) != list(launch):
         return None, None
@@ -285,9 +292,9 @@ def plain_bench(args, workload, K, W, extras=True, cpu=True):
     C, P = count_pairs_run(ps, restart, W, K, args.stats_every)
 
     stages = stage_table(stage_ms, N, G, C, P)
-    roof = roofline(stage_ms, N, G, C, P)
+    roof = roofline(stage_ms, N, G, C, P, batch)
     window = {"steps": K, "warmup": W, "autotuned": tuned is not None, "graph": use_graph}
-    si = STAGES.index(next(k for k, v in STAGE_KERNEL.items() if v == roof["kernel"]))
+    si = STAGES.index(next(k for k, v in (BATCH_KERNEL if batch else STAGE_KERNEL).items() if v == roof["kernel"]))
     roof["traffic"], src = traffic_for(workload, roof["kernel"], window, (thr[si], ps.StagePpt(si)))
     if src:
         roof["traffic_source"] = src
@@ -607,7 +614,8 @@ def summarize(args):
     if args.json:
         with open(args.json, "w") as fh:
             cfg = line["config"]
-            launch = {STAGE_KERNEL[st]: [cfg["stage_threads"][i], cfg.get("stage_ppt", [1] * 6)[i]]
+            names = BATCH_KERNEL if cfg["workload"].startswith("batch") else STAGE_KERNEL
+            launch = {names[st]: [cfg["stage_threads"][i], cfg.get("stage_ppt", [1] * 6)[i]]
                       for i, st in enumerate(STAGES)}
             json.dump({"workload": cfg["workload"].split(":")[0], "window": win, "launch": launch,
                        "bytes_per_launch": traffic,

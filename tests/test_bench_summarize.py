@@ -75,3 +75,6 @@ def test_traffic_attaches_only_to_the_same_window_and_launch(tmp_path, monkeypat
     assert bench.traffic_for("dambreak_4m", "computeForce", win, (128, 1)) == (None, None)
     assert bench.traffic_for("dambreak_4m", "computeForce", dict(win, steps=200), (256, 1)) == (None, None)
     assert bench.traffic_for("dambreak_4m", "computePressure", win, (256, 1)) == (None, None)
+    # block sizes pinned for the counter runs instead of tuned: same window
+    assert bench.traffic_for("dambreak_4m", "computeForce", dict(win, autotuned=False), (256, 1))[0] == 500000000
+    assert bench.traffic_for("dambreak_4m", "computeForce", dict(win, graph=False), (256, 1)) == (None, None)
