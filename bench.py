@@ -55,6 +55,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (and matrix) peak
+NUM_CU, NUM_XCD, NUM_SIMD = 256, 8, 1024  # MI355X: per-CU counters are summed over CUs, GRBM_GUI_ACTIVE over XCDs
 
 # Fluids 5.0 at 4,000,000 particles on its default scene: 160 fps on an RTX 4090 (README.md:12-19)
 REF_BEACH_RTX4090 = 160 * 4_000_000
@@ -75,8 +76,12 @@ CONFIGURED = {"dambreak_4m": (20, 200)}
 
 
 def flops_per_particle(stage, C, P):
+    # The candidate tests (8 flops per candidate) are done once per step, by the
+    # density stage, whose in-radius bitmask the force stage walks: the force
+    # stage is credited with its pair terms only (31 per pair, the pressure
+    # term hoisted), so that no flop of the step is counted twice.
     return {"insert": 6.0, "prefix": 0.0, "sort": 0.0, "pressure": 8 * C + 5 * P + 2,
-            "force": 8 * C + 31 * P + 2, "advance": 60.0}[stage]
+            "force": 31 * P + 2, "advance": 60.0}[stage]
 
 
 def describe(name, sc, N, G):
@@ -225,6 +230,16 @@ def traffic_for(workload, kernel, window, launch):
     return round(t["bytes_per_launch"][kernel]), "profiles/" + os.path.basename(tf)
 
 
+def limits_for(workload, kernel, window, launch):
+    """The hardware limits of `kernel` measured by counters in the committed
+    summary of this workload (texture-path busy fraction, VALU issue rate), on
+    the same terms as traffic_for."""
+    if traffic_for(workload, kernel, window, launch)[0] is None:
+        return None
+    with open(os.path.join(ROOT, "profiles", TRAFFIC_FILES[workload])) as fh:
+        return json.load(fh).get("limits", {}).get(kernel)
+
+
 def timed_steps(ps, K, use_graph):
     """K steps in one graph-timed region, bracketed by fxMark dispatches (tags 1 and 2)."""
     ps.Synchronize()
@@ -298,6 +313,9 @@ def plain_bench(args, workload, K, W, extras=True, cpu=True):
     roof["traffic"], src = traffic_for(workload, roof["kernel"], window, (thr[si], ps.StagePpt(si)))
     if src:
         roof["traffic_source"] = src
+        lim = limits_for(workload, roof["kernel"], window, (thr[si], ps.StagePpt(si)))
+        if lim:
+            roof["limits"] = lim
 
     out = {
         "metric": ("particle-steps/sec at 4M particles (1 GPU); fwd+adjoint step time vs plain"
@@ -594,6 +612,7 @@ def summarize(args):
             f, w = np.mean(v["FETCH_SIZE"]), np.mean(v["WRITE_SIZE"])
             traffic[k] = (f + w) * 1024.0
             out.append(f"| {k} | {len(v['FETCH_SIZE'])} | {f:.0f} | {w:.0f} | {traffic[k] / 1e6:.1f} |")
+    limits, vdur = {}, {}
     if args.counters:
         # any other counters: mean per dispatch over the window, per kernel
         per = {}
@@ -604,6 +623,22 @@ def summarize(args):
                 if a2 < int(r["Dispatch_Id"]) < b2:
                     per.setdefault(_base(r["Kernel_Name"]), {}).setdefault(r["Counter_Name"], []).append(
                         float(r["Counter_Value"]))
+        # busy fraction of the texture-data path and VALU issue rate per SIMD
+        for path in args.counters:
+            rr = _rows(path)
+            a2, b2 = _window(rr)
+            for r in rr:
+                if r.get("Counter_Name") == "SQ_INSTS_VALU" and a2 < int(r["Dispatch_Id"]) < b2:
+                    vdur.setdefault(_base(r["Kernel_Name"]), []).append(
+                        (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9)
+        for k, v in per.items():
+            lim = {}
+            if "TD_TD_BUSY_sum" in v and "GRBM_GUI_ACTIVE" in v:
+                lim["td_busy"] = round(float(np.mean(v["TD_TD_BUSY_sum"]) / NUM_CU / (np.mean(v["GRBM_GUI_ACTIVE"]) / NUM_XCD)), 4)
+            if "SQ_INSTS_VALU" in v and vdur.get(k):
+                lim["valu_per_simd_s"] = float(f"{np.mean(v['SQ_INSTS_VALU']) / NUM_SIMD / np.mean(vdur[k]):.4g}")
+            if lim:
+                limits[k] = lim
         names = sorted({c for v in per.values() for c in v})
         out.append("\nCounters per dispatch in the window (mean):\n")
         out.append("| kernel | dispatches | " + " | ".join(names) + " |\n|---|---:|" + "---:|" * len(names))
@@ -618,7 +653,7 @@ def summarize(args):
             launch = {names[st]: [cfg["stage_threads"][i], cfg.get("stage_ppt", [1] * 6)[i]]
                       for i, st in enumerate(STAGES)}
             json.dump({"workload": cfg["workload"].split(":")[0], "window": win, "launch": launch,
-                       "bytes_per_launch": traffic,
+                       "bytes_per_launch": traffic, "limits": limits,
                        "window_avg_us": {k: float(np.mean(v)) for k, v in dur.items()},
                        "fetch_factor": args.fetch_factor, "write_factor": args.write_factor,
                        "line_value": line["value"], "line_ms_per_step": line["ms_per_step"]}, fh, indent=1)

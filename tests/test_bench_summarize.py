@@ -78,3 +78,35 @@ def test_traffic_attaches_only_to_the_same_window_and_launch(tmp_path, monkeypat
     # block sizes pinned for the counter runs instead of tuned: same window
     assert bench.traffic_for("dambreak_4m", "computeForce", dict(win, autotuned=False), (256, 1))[0] == 500000000
     assert bench.traffic_for("dambreak_4m", "computeForce", dict(win, graph=False), (256, 1)) == (None, None)
+
+
+def test_summarize_derives_the_counter_limits(tmp_path):
+    """td_busy = TD_TD_BUSY_sum / 256 CUs over GRBM_GUI_ACTIVE / 8 XCDs; VALU
+    issue = SQ_INSTS_VALU / 1024 SIMDs over the dispatch's duration; both
+    averaged over the window's dispatches only."""
+    seq = [("computeForce", 9)] + [("fxMark", 1), ("computeForce", 1), ("computeForce", 1), ("fxMark", 1)]
+    c1, c2 = [], []
+    for i, (k, scale) in enumerate(seq):
+        t0 = 1000 * i
+        base = {"Dispatch_Id": i + 1, "Kernel_Name": k, "Start_Timestamp": t0, "End_Timestamp": t0 + 2000 * scale}
+        # inside the window: 2 us per dispatch, 1.024e6 VALU -> 1000 per SIMD -> 5e8 per SIMD per second
+        c1.append(dict(base, Counter_Name="SQ_INSTS_VALU", Counter_Value=1.024e6 * scale))
+        # TD busy 0.9 of 1000 cycles per XCD on every CU
+        c2.append(dict(base, Counter_Name="TD_TD_BUSY_sum", Counter_Value=900.0 * 256 * scale))
+        c2.append(dict(base, Counter_Name="GRBM_GUI_ACTIVE", Counter_Value=1000.0 * 8 * scale))
+    f = ["Dispatch_Id", "Kernel_Name", "Start_Timestamp", "End_Timestamp", "Counter_Name", "Counter_Value"]
+    write_csv(tmp_path / "c1.csv", c1, f)
+    write_csv(tmp_path / "c2.csv", c2, f)
+    write_csv(tmp_path / "fw.csv", [dict(r, Counter_Name=n, Counter_Value=1.0) for r in c1 for n in ("FETCH_SIZE",)], f)
+    write_csv(tmp_path / "ww.csv", [dict(r, Counter_Name="WRITE_SIZE", Counter_Value=1.0) for r in c1], f)
+    line = {"value": 1.0, "ms_per_step": 0.15, "config": {
+        "workload": "dambreak_4m: x", "stage_threads": [64, 128, 1024, 256, 256, 128], "stage_ppt": [1] * 6,
+        "timed_window": {"steps": 2, "warmup": 5, "autotuned": False, "graph": True}}}
+    (tmp_path / "line.json").write_text(json.dumps(line) + "\n")
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--summarize", "--line",
+                          str(tmp_path / "line.json"), "--fetch", str(tmp_path / "fw.csv"), "--write",
+                          str(tmp_path / "ww.csv"), "--counters", str(tmp_path / "c1.csv"), str(tmp_path / "c2.csv"),
+                          "--json", str(tmp_path / "t.json")], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    lim = json.loads((tmp_path / "t.json").read_text())["limits"]["computeForce"]
+    assert lim == {"td_busy": 0.9, "valu_per_simd_s": 5e8}
