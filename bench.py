@@ -33,7 +33,7 @@ Other workloads (--workload): dambreak_4m_rest (the same 4M particles at rest
 spacing), batch64 (64-scene batch, with the same scenes stepped one at a time
 beside it), adjoint_262k / adjoint_batch32 (forward+adjoint rollouts), ref_beach (the
 reference's default scene, the only one with a published rate:
-vs_baseline = value / 6.4e8, RTX 4090, README.md:12-19).  The default line
+vs_baseline = value / 6.4e8, RTX 4090, the reference's README.md:12-19).  The default line
 also carries batch64, adjoint_262k and adjoint_batch32 summaries ("extras";
 --no-extras leaves them out).
 
@@ -57,7 +57,7 @@ HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (and matrix) peak
 NUM_CU, NUM_XCD, NUM_SIMD = 256, 8, 1024  # MI355X: per-CU counters are summed over CUs, GRBM_GUI_ACTIVE over XCDs
 
-# Fluids 5.0 at 4,000,000 particles on its default scene: 160 fps on an RTX 4090 (README.md:12-19)
+# Fluids 5.0 at 4,000,000 particles on its default scene: 160 fps on an RTX 4090 (the reference's README.md:12-19)
 REF_BEACH_RTX4090 = 160 * 4_000_000
 
 TRAFFIC_FILES = {"dambreak_4m": "traffic_dambreak_4M_literal.json", "dambreak_4m_rest": "traffic_dambreak_4M_rest.json",
@@ -330,7 +330,7 @@ def plain_bench(args, workload, K, W, extras=True, cpu=True):
         "ms_per_step": round(ms_step, 5),
         "higher_is_better": True,
         "scaling": "weak",
-        # the reference's published rate (README.md:12-19, RTX 4090) is measured on its
+        # the reference's published rate (its README.md:12-19, RTX 4090) is measured on its
         # default beach scene: only that workload is compared with it
         "vs_baseline": round(value / REF_BEACH_RTX4090, 3) if workload == "ref_beach" else None,
         "dtype": "f32",
@@ -514,7 +514,7 @@ def extras(args):
     r = plain_bench(args, "ref_beach", 200, 20, extras=False, cpu=False)
     ex["ref_beach"] = {"value": r["value"], "ms_per_step": r["ms_per_step"], "steps": 200, "warmup": 20,
                        "particles": r["config"]["particles"], "vs_rtx4090_published": r["vs_baseline"],
-                       "published": "160 fps = 6.4e8 particle-steps/s on an RTX 4090 (README.md:12-19)",
+                       "published": "160 fps = 6.4e8 particle-steps/s on an RTX 4090 (the reference's README.md:12-19)",
                        "roofline": {k: r["roofline"][k] for k in ("bound", "frac", "kernel", "kernel_ms")}}
     progress("extras: batch64")
     b = plain_bench(args, "batch64", 100, args.warmup, extras=False, cpu=False)

@@ -1,6 +1,6 @@
 // scene_xml.cpp -- scene files in the reference's scene.xml schema -> fx_scene.
 //
-// The schema (README.md "Scene parameters"; the example file
+// The schema (the reference's README.md "Scene parameters"; the example file
 // fluids5.0/assets/scene.xml) is one <Fluid> element followed by
 // <Scene id="k"> elements, each a flat list of <Key> value </Key> parameters.
 // Any parameter may appear in either section and a scene's values override

@@ -2,7 +2,7 @@
 
 A thin mirror of the C ABI's reader (fx_scene_from_xml / fx_scene_xml_ids in
 libfluidsx.so, fluids3_amd/csrc/scene_xml.cpp), so that a C++ host and Python
-read a file the same way.  The schema (README.md "Scene parameters"; the
+read a file the same way.  The schema (the reference's README.md "Scene parameters"; the
 reference's example file fluids5.0/assets/scene.xml) is one <Fluid> element
 followed by <Scene id="k"> elements, each holding parameters as text
 elements; a scene's values override the Fluid section's (FluidSystem::ParseXML

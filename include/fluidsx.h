@@ -82,7 +82,7 @@ float fx_scene_spacing(const fx_scene* scene);
  * number of particles written. */
 int fx_scene_fill(const fx_scene* scene, float* pos, uint32_t* clr, int32_t n);
 
-/* Scene files in the reference's scene.xml schema (README.md "Scene
+/* Scene files in the reference's scene.xml schema (its README.md "Scene
  * parameters", fluids5.0/assets/scene.xml): a <Fluid> section, then
  * <Scene id="k"> sections; a scene's keys override the Fluid section's, as
  * FluidSystem::ParseXML applies "Fluid" and then the scene
