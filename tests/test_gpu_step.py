@@ -398,6 +398,96 @@ def test_crowded_cell_is_ordered(ps):
     compare_state(g, oracle.run(p, fresh_oracle(p, s0), 2, threads=THREADS))
 
 
+def trim_probe_scene(p, pos):
+    """Probes and candidates for the row-trim test, written over the first
+    particles of `pos`: each probe in a row of its own, at x fractions k/16 of
+    its cell (the radius then falls on a slab boundary of the row's first and
+    last cells) shifted by -3..3 ulps, its three row cells crowded (20
+    members: slab starts exist), and candidates of its row at x distance
+    R (1 - d), d in {0, 1e-7, 2e-7, 5e-7, 1e-6}, and the floats either side, on
+    both sides.  Returns the probe indices and, per probe, its candidates."""
+    F = np.float32
+    rng = np.random.default_rng(5)
+    g0 = np.array(p.gridMin.tolist(), np.float64)
+    w = 1.0 / np.array(p.gridDelta.tolist(), np.float64)  # cell edge per axis
+    R = float(np.sqrt(F(p.rd2)))  # radius in world units
+    at = lambda gx, gy, gz: (g0 + np.array([gx, gy, gz]) * w).astype(F)
+    k = 0
+    probes, cands = [], []
+    rows = [(cy, cz) for cy in range(2, 26, 3) for cz in range(2, 26, 2)]
+    shots = [(f / 16.0, u) for f in (1, 5, 8, 11, 15) for u in (-3, -1, 0, 1, 3)]
+    for (cy, cz), (frac, ulps) in zip(rows, shots):
+        for cx in (9, 10, 11):
+            for _ in range(20):
+                pos[k] = at(cx + rng.uniform(0.05, 0.95), cy + rng.uniform(0.05, 0.95), cz + rng.uniform(0.05, 0.95))
+                k += 1
+        probe = at(10 + frac, cy + 0.5, cz + 0.5)
+        probe[0] = probe[0] + F(ulps) * np.spacing(probe[0])
+        pos[k] = probe
+        probes.append(k)
+        k += 1
+        mine = []
+        for d in (0.0, 1e-7, 2e-7, 5e-7, 1e-6):
+            for side in (-1.0, 1.0):
+                q = F(float(probe[0]) + side * R * (1.0 - d))
+                for qx in (np.nextafter(q, F(-1e9)), q, np.nextafter(q, F(1e9))):
+                    pos[k] = probe
+                    pos[k, 0] = qx
+                    mine.append(k)
+                    k += 1
+        cands.append(mine)
+    return probes, cands
+
+
+def test_row_trim_at_the_radius(ps):
+    """The density stage skips the x slabs of a stencil row's first and last
+    cells that lie beyond a particle's radius (a bound that allows for the
+    rounding of the cell coordinate).  Probes put the radius on slab
+    boundaries, to within a few ulps, with candidates of their own row at
+    distances R (1 - d) and the floats either side.  Some of the candidates
+    inside the radius lie in slabs that a bound without that allowance would
+    skip (checked below, in float32 as the kernel computes it), so a bound that
+    is too tight changes a probe's density: one step must equal the oracle
+    bit for bit."""
+    F = np.float32
+    sc = scenes.dambreak_16k()
+    ps.Restart(sc)
+    p = ps.params
+    pos = ps.state()["pos"].copy()
+    probes, cands = trim_probe_scene(p, pos)
+    # the construction's power: inside candidates in a slab that the bare bound
+    # (sqrt(rd2) in grid units, no allowance) would skip
+    gmin, dl, rd2 = F(p.gridMin.x), F(p.gridDelta.x), F(p.rd2)
+    gxf = lambda x: (F(x) - gmin) * dl
+    bare = F(np.sqrt(rd2)) * dl
+    sensitive = 0
+    for q, mine in zip(probes, cands):
+        gp = gxf(pos[q, 0])
+        cx = int(gp)
+        lo = min(max(int(np.ceil(F(16) * (gp - F(cx - 1) - bare) - F(1))), 0), 16)
+        hi = min(max(int(np.floor(F(16) * (gp - F(cx + 1) + bare))) + 1, 0), 16)
+        for j in mine:
+            dx = F(pos[q, 0] - pos[j, 0])
+            if not dx * dx < rd2:
+                continue
+            g = gxf(pos[j, 0])
+            slab = min(int((g - F(int(g))) * F(16)), 15)
+            if (int(g) == cx - 1 and slab < lo) or (int(g) == cx + 1 and slab >= hi):
+                sensitive += 1
+    assert sensitive > 0
+    ps.Commit(abi.FPOS, pos)
+    s0 = ps.state()
+    cells = oracle.run(p, fresh_oracle(p, s0), 1, oracle.STAGE_INSERT)
+    for q in probes:  # every row cell of every probe has slab starts (more than 16 members)
+        c = int(cells["gcell"][q])
+        assert all(cells["cnt"][c + dx] > 16 for dx in (-1, 0, 1))
+    ps.Run(1)
+    g = ps.state()
+    ref = oracle.run(p, fresh_oracle(p, s0), 1, threads=THREADS)
+    compare_state(g, ref)
+    print(f"row trim: {len(probes)} probes, {sensitive} inside candidates in slabs a bare bound would skip")
+
+
 @pytest.mark.parametrize("n", [1, 2, 1023, 1024, 1025, 2047, 2048, 4097, 1024 * 1024 - 1, 1024 * 1024,
                                1024 * 1024 + 1, 3 * 1024 * 1024 + 7])
 def test_scan_lengths(ps, n):
