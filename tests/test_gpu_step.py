@@ -554,6 +554,26 @@ def test_dambreak_4m_full_size(ps):
         assert e.max() < (1e-5 if k == "press" else 1e-1), k
 
 
+def test_dambreak_4m_benchmark_window_bitexact(ps):
+    """The benchmark's own run at 4,000,000 particles, 5 eager warm-up steps
+    then 20 steps as the captured step graph, against 25 oracle steps from the
+    same initial state: the whole state bit for bit (cells, counts, offsets,
+    membership, ids, density, force, positions and velocities)."""
+    sc = scenes.dambreak_4m()
+    ps.Restart(sc)
+    p = ps.params
+    s0 = ps.state()
+    ps.Run(5)
+    ps.CaptureGraph()
+    ps.LaunchGraph(10)  # two steps per launch
+    ps.Synchronize()
+    g = ps.state()
+    print("4M: 25 GPU steps done; 25 oracle steps", flush=True)
+    ref = oracle.run(p, fresh_oracle(p, s0), 25, threads=THREADS)
+    assert g["time"] == ref["time"]
+    compare_state(g, ref)
+
+
 def test_autotune_keeps_state_and_results(ps):
     """The tuner picks a listed block size per stage, leaves the state it
     started from, and a rollout under the tuned sizes is bit-identical to
