@@ -34,6 +34,7 @@
 #include <climits>
 
 #include "fluidsx_params.h"
+#include "fluidsx_trig.h"
 
 typedef unsigned int uint;
 
@@ -301,7 +302,7 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) 
     ax += nx - ve.x * fric; ay += ny - ve.y * fric; az += nz - ve.z * fric;
   }
   const float sarg = A.time * P.bound_wall_freq + pos.z / 400.0f;
-  const float wall = (float)(((double)sinf(sarg) * 0.5 + 0.5) * (double)P.bound_wall_force);
+  const float wall = (float)(((double)fx_sinf(sarg) * 0.5 + 0.5) * (double)P.bound_wall_force);
   diff = rad - (pos.x - (bmin.x + wall)) * ss;
   const float diff3 = diff;
   const bool w3 = diff > FX_EPSILON;
@@ -385,7 +386,7 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) 
   if (w3) {
     px -= gax * (wall * stiff * ss);
     if (P.bound_wall_force != 0.0f) {
-      const float dwall = cosf(sarg) * (0.5f / 400.0f) * P.bound_wall_force;
+      const float dwall = fx_cosf(sarg) * (0.5f / 400.0f) * P.bound_wall_force;
       pz += gax * (stiff * dwall * (diff3 + wall * ss));
     }
     ex -= damp * gax;

@@ -22,6 +22,7 @@
 #include <type_traits>
 
 #include "fluidsx_params.h"
+#include "fluidsx_trig.h"
 
 typedef unsigned int uint;
 
@@ -1610,7 +1611,7 @@ __device__ __forceinline__ void advance_one(const FParams_t& P, int i, float tim
   }
   // X-min, moving wall; its stiffness term is scaled by 'wall'
   const float sarg = time * P.bound_wall_freq + pos.z / 400.0f;
-  const float wall = (float)(((double)sinf(sarg) * 0.5 + 0.5) * (double)P.bound_wall_force);
+  const float wall = (float)(((double)fx_sinf(sarg) * 0.5 + 0.5) * (double)P.bound_wall_force);
   diff = rad - (pos.x - (bmin.x + wall)) * ss;
   if (diff > FX_EPSILON) {
     nx = 1.0f; ny = 0.0f; nz = 0.0f;

@@ -1,0 +1,63 @@
+/* fluidsx_trig.h -- sine and cosine of a float, written out as a fixed
+ * sequence of IEEE operations, so that the kernels and any restatement of
+ * them on a CPU compute the same bits.  Used for the moving X-min wall of the
+ * advance stage (fluids5.0/src/particles.cu:275: sin(time * freq + z / 400))
+ * and its adjoint.  The C library's sinf and the GPU math library's each
+ * round correctly most of the time but not always the same way; one ulp of
+ * difference in the wall position is enough to make two otherwise identical
+ * trajectories part.
+ *
+ * Method (textbook argument reduction and Taylor series):
+ *   k = rint(x * 2/pi);  r = x - k * pi/2, in double, with pi/2 the sum of two
+ *   doubles and each product folded in by an fma (Cody and Waite's split);
+ *   r rounded to float, |r| <= pi/4 (plus rounding);
+ *   sin r = r - r^3/3! + r^5/5! - r^7/7! + r^9/9!   (truncation < 1.8e-9)
+ *   cos r = 1 - r^2/2! + ... - r^10/10!              (truncation < 1.2e-10)
+ *   evaluated in float by Horner's rule with fmaf; k mod 4 selects sin or cos
+ *   of r and the sign.  The constants are the double nearest to 2/pi, pi/2
+ *   and pi/2 minus it (from a 60-digit pi), and the floats nearest to 1/n!.
+ * Accuracy against a double-precision sine and cosine: within 2 ulps of the
+ * float result for |x| up to 1e6 (checked by tests/test_oracle_trig.py).
+ * Infinite and NaN arguments give NaN; sin(-0) gives +0.
+ */
+#ifndef FLUIDSX_TRIG_H
+#define FLUIDSX_TRIG_H
+
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define FX_TRIG_FN __host__ __device__ inline
+#else
+#define FX_TRIG_FN inline
+#endif
+
+/* sin(x) for shift 0, cos(x) = sin(x + pi/2) for shift 1 */
+FX_TRIG_FN float fx_trig_shifted(float x, int shift) {
+  if (!(fabsf(x) <= 0x1.fffffep+127f)) return x - x; /* NaN for +-inf and NaN */
+  const double xd = (double)x;
+  const double k = rint(xd * 0x1.45f306dc9c883p-1);     /* 2/pi */
+  double rd = fma(-k, 0x1.921fb54442d18p+0, xd);         /* pi/2, leading double */
+  rd = fma(-k, 0x1.1a62633145c07p-54, rd);                /* pi/2, the rest */
+  const float r = (float)rd;
+  /* k mod 4 without converting a huge k to int: k - 4 floor(k/4) is exact */
+  const int q = ((int)(k - 4.0 * floor(k * 0.25)) + shift) & 3;
+  const float r2 = r * r;
+  float s = fmaf(0x1.71de3ap-19f, r2, -0x1.a01a02p-13f); /* 1/9!, -1/7! */
+  s = fmaf(s, r2, 0x1.111112p-7f);                        /* 1/5! */
+  s = fmaf(s, r2, -0x1.555556p-3f);                       /* -1/3! */
+  s = fmaf(r * r2, s, r);
+  float c = fmaf(-0x1.27e4fcp-22f, r2, 0x1.a01a02p-16f); /* -1/10!, 1/8! */
+  c = fmaf(c, r2, -0x1.6c16c2p-10f);                      /* -1/6! */
+  c = fmaf(c, r2, 0x1.555556p-5f);                        /* 1/4! */
+  c = fmaf(c, r2, -0.5f);                                 /* -1/2! */
+  c = fmaf(r2, c, 1.0f);
+  const float v = (q & 1) ? c : s;
+  return (q & 2) ? -v : v;
+}
+
+FX_TRIG_FN float fx_sinf(float x) { return fx_trig_shifted(x, 0); }
+FX_TRIG_FN float fx_cosf(float x) { return fx_trig_shifted(x, 1); }
+
+#undef FX_TRIG_FN
+
+#endif /* FLUIDSX_TRIG_H */

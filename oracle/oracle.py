@@ -62,6 +62,8 @@ def lib():
         L.orc_splitmix64.argtypes = [C.c_uint64]
         L.orc_emit.restype = C.c_int
         L.orc_emit.argtypes = [C.c_int, C.c_float, C.c_int, P, P, P, P]
+        L.orc_trig.restype = C.c_int
+        L.orc_trig.argtypes = [C.c_int, P, P, P]
         L.orc_grad_mixed.restype = C.c_int
         L.orc_grad_mixed.argtypes = [P, C.c_int, P, P, P, P, P, C.c_float, C.c_int, P, C.c_int, P, P, P, P, P, P]
         _lib = L
@@ -256,4 +258,13 @@ def emit(state, frame, count):
     force = np.array(state.get("force", np.zeros((n, 3))), np.float32, order="C", copy=True)
     lib().orc_emit(n, float(frame), int(count), _ptr(pos), _ptr(vel), _ptr(veval), _ptr(force))
     return pos, vel, veval, force
+
+
+def trig(x):
+    """The moving wall's sine and cosine (the kernels' own, fluidsx_trig.h) of float32 x."""
+    x = np.ascontiguousarray(x, np.float32)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    lib().orc_trig(x.size, _ptr(x), _ptr(s), _ptr(c))
+    return s, c
 

@@ -20,6 +20,8 @@
 //   orc_sample            volume splat              pinned: single-particle closed form
 //   orc_emit              emitter respawn           pinned: SplitMix64 known answer, distinct
 //                                                     targets, box lattice, velocity
+//   orc_trig              the wall's sine/cosine    pinned: against double-precision sin
+//                                                     and cos within 2 ulps
 //   orc_grad_*            adjoint of a rollout      pinned (FP64): central differences of
 //                                                     orc_loss_f64 in every input kind,
 //                                                     a two-particle closed form
@@ -315,6 +317,16 @@ int orc_emit(int n, float frame, int emit, float* pos, float* vel, float* veval,
     vel[3 * j] = 5.0f;
     vel[3 * j + 1] = -2.0f;
     vel[3 * j + 2] = 0.0f;
+  }
+  return 0;
+}
+
+// The moving wall's sine and cosine (fluidsx_trig.h, the kernels' own) of n
+// floats, for the accuracy check.
+int orc_trig(int n, const float* x, float* s, float* c) {
+  for (int i = 0; i < n; i++) {
+    s[i] = fx_sinf(x[i]);
+    c[i] = fx_cosf(x[i]);
   }
   return 0;
 }

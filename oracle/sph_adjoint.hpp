@@ -142,7 +142,7 @@ void advance_bwd_one(const FParams_t& P, const TapeStep<R>& T, int s, R dt, R ss
     ax += nx - ve.x * fric; ay += ny - ve.y * fric; az += nz - ve.z * fric;
   }
   const R sarg = T.time * (R)P.bound_wall_freq + pos.z / R(400);
-  const R wall = (R)(((double)std::sin(sarg) * 0.5 + 0.5) * (double)P.bound_wall_force);
+  const R wall = (R)(((double)orc_sin(sarg) * 0.5 + 0.5) * (double)P.bound_wall_force);
   diff = rad - (pos.x - (bminx + wall)) * ss;
   const R diff3 = diff;
   w3 = diff > eps;
@@ -225,7 +225,7 @@ void advance_bwd_one(const FParams_t& P, const TapeStep<R>& T, int s, R dt, R ss
   if (w3) {
     px -= gax * (wall * stiff * ss);
     if (P.bound_wall_force != 0.0f) {
-      const R dwall = std::cos(sarg) * (R(0.5) / R(400)) * (R)P.bound_wall_force;
+      const R dwall = orc_cos(sarg) * (R(0.5) / R(400)) * (R)P.bound_wall_force;
       pz += gax * (stiff * dwall * (diff3 + wall * ss));
     }
     ex -= damp * gax;

@@ -28,8 +28,17 @@
 #include <vector>
 
 #include "../include/fluidsx_params.h"
+#include "../include/fluidsx_trig.h"
 
 namespace oracle {
+
+// Sine and cosine of the moving wall: the float step takes the same written-out
+// sequence of IEEE operations as the kernels (fluidsx_trig.h), so that both
+// produce the same bits; the double step takes the C library's.
+inline float orc_sin(float x) { return fx_sinf(x); }
+inline double orc_sin(double x) { return std::sin(x); }
+inline float orc_cos(float x) { return fx_cosf(x); }
+inline double orc_cos(double x) { return std::cos(x); }
 
 template <class R>
 struct V3 {
@@ -294,7 +303,7 @@ void advance_one(const FParams_t& P, State<R>& S, int i, R time, R dt, R ss) {
     az += nz - ve.z * fric;
   }
   const R sarg = time * (R)P.bound_wall_freq + pos.z / R(400);
-  const R wall = (R)(((double)std::sin(sarg) * 0.5 + 0.5) * (double)P.bound_wall_force);
+  const R wall = (R)(((double)orc_sin(sarg) * 0.5 + 0.5) * (double)P.bound_wall_force);
   diff = rad - (pos.x - (bminx + wall)) * ss;
   if (diff > eps) {
     nx = 1; ny = 0; nz = 0;
