@@ -34,7 +34,7 @@
 #include <climits>
 
 #include "fluidsx_params.h"
-#include "fluidsx_trig.h"
+#include "fx_trig.hpp"
 
 typedef unsigned int uint;
 

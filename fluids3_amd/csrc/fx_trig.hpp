@@ -1,4 +1,4 @@
-/* fluidsx_trig.h -- sine and cosine of a float, written out as a fixed
+/* fx_trig.hpp -- sine and cosine of a float, written out as a fixed
  * sequence of IEEE operations, so that the kernels and any restatement of
  * them on a CPU compute the same bits.  Used for the moving X-min wall of the
  * advance stage (fluids5.0/src/particles.cu:275: sin(time * freq + z / 400))
@@ -20,8 +20,8 @@
  * float result for |x| up to 1e6 (checked by tests/test_oracle_trig.py).
  * Infinite and NaN arguments give NaN; sin(-0) gives +0.
  */
-#ifndef FLUIDSX_TRIG_H
-#define FLUIDSX_TRIG_H
+#ifndef FX_TRIG_HPP
+#define FX_TRIG_HPP
 
 #include <math.h>
 
@@ -60,4 +60,4 @@ FX_TRIG_FN float fx_cosf(float x) { return fx_trig_shifted(x, 1); }
 
 #undef FX_TRIG_FN
 
-#endif /* FLUIDSX_TRIG_H */
+#endif /* FX_TRIG_HPP */

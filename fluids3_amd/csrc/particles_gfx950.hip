@@ -22,7 +22,7 @@
 #include <type_traits>
 
 #include "fluidsx_params.h"
-#include "fluidsx_trig.h"
+#include "fx_trig.hpp"
 
 typedef unsigned int uint;
 

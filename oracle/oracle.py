@@ -261,7 +261,7 @@ def emit(state, frame, count):
 
 
 def trig(x):
-    """The moving wall's sine and cosine (the kernels' own, fluidsx_trig.h) of float32 x."""
+    """The moving wall's sine and cosine (the kernels' own, fluids3_amd/csrc/fx_trig.hpp) of float32 x."""
     x = np.ascontiguousarray(x, np.float32)
     s = np.empty_like(x)
     c = np.empty_like(x)

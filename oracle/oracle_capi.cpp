@@ -321,7 +321,7 @@ int orc_emit(int n, float frame, int emit, float* pos, float* vel, float* veval,
   return 0;
 }
 
-// The moving wall's sine and cosine (fluidsx_trig.h, the kernels' own) of n
+// The moving wall's sine and cosine (fx_trig.hpp, the kernels' own) of n
 // floats, for the accuracy check.
 int orc_trig(int n, const float* x, float* s, float* c) {
   for (int i = 0; i < n; i++) {

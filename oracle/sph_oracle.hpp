@@ -28,12 +28,12 @@
 #include <vector>
 
 #include "../include/fluidsx_params.h"
-#include "../include/fluidsx_trig.h"
+#include "../fluids3_amd/csrc/fx_trig.hpp"
 
 namespace oracle {
 
 // Sine and cosine of the moving wall: the float step takes the same written-out
-// sequence of IEEE operations as the kernels (fluidsx_trig.h), so that both
+// sequence of IEEE operations as the kernels (fluids3_amd/csrc/fx_trig.hpp), so that both
 // produce the same bits; the double step takes the C library's.
 inline float orc_sin(float x) { return fx_sinf(x); }
 inline double orc_sin(double x) { return std::sin(x); }

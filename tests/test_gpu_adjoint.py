@@ -101,7 +101,7 @@ def test_adjoint_walls_slope_friction_moving_wall(ps):
     """Every wall branch (sloped floor with friction, moving X- wall, X+/Z
     walls, ceiling), initial velocities nonzero: GPU against the FP32 oracle,
     bit for bit (the moving wall's sine and cosine are the kernels' own,
-    fluidsx_trig.h, which the oracle shares)."""
+    fluids3_amd/csrc/fx_trig.hpp, which the oracle shares)."""
     sc = scenes.dam_break(600, (-12, 0, -12), (12, 14, 12), (-11.5, 0.5, -11.5), (11.5, 13.5, 11.5))
     sc.bound_slope = 0.12
     sc.bound_friction = 0.3

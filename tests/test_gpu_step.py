@@ -612,7 +612,7 @@ def test_reference_default_scene(ps):
     """The reference's default beach scene (cell 0.75 h under a 27-cell
     stencil, moving X- wall, sloped floor), 20,000 particles, 40 steps, bit
     for bit with the FP32 oracle: the wall's sine is the kernels' own
-    (fluidsx_trig.h), which the oracle shares."""
+    (fluids3_amd/csrc/fx_trig.hpp), which the oracle shares."""
     sc = scenes.ref_beach(20000)
     ps.Restart(sc)
     p = ps.params

@@ -1,4 +1,4 @@
-"""The moving wall's sine and cosine (include/fluidsx_trig.h), which the
+"""The moving wall's sine and cosine (fluids3_amd/csrc/fx_trig.hpp), which the
 kernels and the oracle share so that both compute the same bits: against
 double-precision sin and cos of the same float argument, within 2 ulps of the
 float result, over small, moderate and large arguments, the neighbourhoods of
