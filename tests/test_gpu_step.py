@@ -574,6 +574,29 @@ def test_dambreak_4m_benchmark_window_bitexact(ps):
     compare_state(g, ref)
 
 
+def test_ref_beach_4m_benchmark_window_bitexact(ps):
+    """The reference's own 4,000,000-particle beach scene (its published rate's
+    scene: moving wall, sloped floor) as the benchmark runs it, 20 eager
+    warm-up steps then steps of the captured graph (the first 80 of the
+    benchmark's 200, to keep the oracle's share of the test under a minute),
+    against 100 oracle steps from the same initial state: the whole state bit
+    for bit.  (All 220 steps of the benchmark's run measured the same.)"""
+    sc = scenes.ref_beach()
+    ps.Restart(sc)
+    p = ps.params
+    assert ps.NumPoints() == 4_000_000 and p.bound_wall_force != 0.0
+    s0 = ps.state()
+    ps.Run(20)
+    ps.CaptureGraph()
+    ps.LaunchGraph(40)  # two steps per launch
+    ps.Synchronize()
+    g = ps.state()
+    print("ref_beach 4M: 100 GPU steps done; 100 oracle steps", flush=True)
+    ref = oracle.run(p, fresh_oracle(p, s0), 100, threads=THREADS)
+    assert g["time"] == ref["time"]
+    compare_state(g, ref)
+
+
 def test_autotune_keeps_state_and_results(ps):
     """The tuner picks a listed block size per stage, leaves the state it
     started from, and a rollout under the tuned sizes is bit-identical to
