@@ -7,15 +7,18 @@
  * difference in the wall position is enough to make two otherwise identical
  * trajectories part.
  *
- * Method (textbook argument reduction and Taylor series):
- *   k = rint(x * 2/pi);  r = x - k * pi/2, in double, with pi/2 the sum of two
- *   doubles and each product folded in by an fma (Cody and Waite's split);
- *   r rounded to float, |r| <= pi/4 (plus rounding);
+ * Method (textbook argument reduction and Taylor series), all in float:
+ *   k = rint(x * 2/pi);  r = x - k * pi/2 with pi/2 = P1 + P2 + P3, three floats
+ *   (Cody and Waite's split), each product folded in by one fmaf, so that the
+ *   first step is exact and r is accurate to about an ulp for |k| up to
+ *   millions; |r| <= pi/4 plus rounding;
  *   sin r = r - r^3/3! + r^5/5! - r^7/7! + r^9/9!   (truncation < 1.8e-9)
  *   cos r = 1 - r^2/2! + ... - r^10/10!              (truncation < 1.2e-10)
- *   evaluated in float by Horner's rule with fmaf; k mod 4 selects sin or cos
- *   of r and the sign.  The constants are the double nearest to 2/pi, pi/2
- *   and pi/2 minus it (from a 60-digit pi), and the floats nearest to 1/n!.
+ *   by Horner's rule with fmaf; k mod 4 (fmodf, exact for any float k)
+ *   selects sin or cos of r and the sign.  The constants are the floats
+ *   nearest to 2/pi and to the successive remainders of pi/2 (from a 60-digit
+ *   pi), and the floats nearest to 1/n!.  No double arithmetic: the device
+ *   runs it in FP32.
  * Accuracy against a double-precision sine and cosine: within 2 ulps of the
  * float result for |x| up to 1e6 (checked by tests/test_oracle_trig.py).
  * Infinite and NaN arguments give NaN; sin(-0) gives +0.
@@ -34,13 +37,11 @@
 /* sin(x) for shift 0, cos(x) = sin(x + pi/2) for shift 1 */
 FX_TRIG_FN float fx_trig_shifted(float x, int shift) {
   if (!(fabsf(x) <= 0x1.fffffep+127f)) return x - x; /* NaN for +-inf and NaN */
-  const double xd = (double)x;
-  const double k = rint(xd * 0x1.45f306dc9c883p-1);     /* 2/pi */
-  double rd = fma(-k, 0x1.921fb54442d18p+0, xd);         /* pi/2, leading double */
-  rd = fma(-k, 0x1.1a62633145c07p-54, rd);                /* pi/2, the rest */
-  const float r = (float)rd;
-  /* k mod 4 without converting a huge k to int: k - 4 floor(k/4) is exact */
-  const int q = ((int)(k - 4.0 * floor(k * 0.25)) + shift) & 3;
+  const float k = rintf(x * 0x1.45f306p-1f);          /* 2/pi */
+  float r = fmaf(-k, 0x1.921fb6p+0f, x);               /* pi/2 = P1 */
+  r = fmaf(-k, -0x1.777a5cp-25f, r);                   /*       + P2 */
+  r = fmaf(-k, -0x1.ee59dap-50f, r);                   /*       + P3 */
+  const int q = ((int)fmodf(k, 4.0f) + shift) & 3;
   const float r2 = r * r;
   float s = fmaf(0x1.71de3ap-19f, r2, -0x1.a01a02p-13f); /* 1/9!, -1/7! */
   s = fmaf(s, r2, 0x1.111112p-7f);                        /* 1/5! */
