@@ -1611,6 +1611,9 @@ __device__ __forceinline__ void advance_one(const FParams_t& P, int i, float tim
   }
   // X-min, moving wall; its stiffness term is scaled by 'wall'
   const float sarg = time * P.bound_wall_freq + pos.z / 400.0f;
+  // in double, as the reference (its 0.5 are double literals) and the oracle:
+  // an FP32 restatement would round (s + 1) force / 2 once where the double
+  // path may round twice
   const float wall = (float)(((double)fx_sinf(sarg) * 0.5 + 0.5) * (double)P.bound_wall_force);
   diff = rad - (pos.x - (bmin.x + wall)) * ss;
   if (diff > FX_EPSILON) {
