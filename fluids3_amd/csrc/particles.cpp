@@ -22,7 +22,7 @@ static const char* kKernelNames[FUNC_MAX] = {
     "computeForceBatch", "advanceParticlesBatch", "fxCountPairsBatch", "sampleParticles",
     "emitParticles",     "fxMaxWords",       "fxMaxWordsBatch",  "fxMark",
     "insertParticlesBatchP", "countingSortIndexBatchP", "countingSortFullBatchP", "computePressureBatchP",
-    "computeForceBatchP", "advanceParticlesBatchP"};
+    "computeForceBatchP", "advanceParticlesBatchP", "fxScanSums", "fxScanApply"};
 
 Particles::Particles() { memset(&m_Params, 0, sizeof(m_Params)); }
 
@@ -468,14 +468,31 @@ int Particles::InsertParticles() {
 
 int Particles::PrefixScanParticles() {
   // particles.cpp:774-815, over gridTotal+1 counts; then the out-of-range
-  // flags, whose exclusive scan ranks the tail bucket in particle order
-  FX_TRY(ScanLevels(m_Accel.gpu(AGRIDCNT), m_Accel.gpu(AGRIDOFF), m_Accel.gpu(AAUXARRAY1),
-                    m_Accel.gpu(AAUXSCAN1), m_Accel.gpu(AAUXARRAY2), m_Accel.gpu(AAUXSCAN2),
-                    m_ScanCells));
-  FX_TRY(ScanLevels(m_Accel.gpu(ATAILFLAG), m_Accel.gpu(ATAILSCAN), m_Accel.gpu(ATAILAUX1),
-                    m_Accel.gpu(ATAILAUXS1), m_Accel.gpu(ATAILAUX2), m_Accel.gpu(ATAILAUXS2), mNumPoints));
+  // flags, whose exclusive scan ranks the tail bucket in particle order.
+  // Two launches per scan (the block totals go to the first auxiliary
+  // level, which holds one entry per 1024 elements); the reference's
+  // three-level prefixSum/prefixFixup beyond FX_SCAN2_MAX_BLOCKS blocks.
+  FX_TRY(ScanTwoLaunch(m_Accel.gpu(AGRIDCNT), m_Accel.gpu(AGRIDOFF), m_Accel.gpu(AAUXARRAY1), m_ScanCells));
+  FX_TRY(ScanTwoLaunch(m_Accel.gpu(ATAILFLAG), m_Accel.gpu(ATAILSCAN), m_Accel.gpu(ATAILAUX1), mNumPoints));
   m_Done = FX_STAGE_PREFIX;
   return FX_OK;
+}
+
+int Particles::ScanTwoLaunch(void* in, void* out, void* sums, int n) {
+  const int nb = (n + FX_SCAN2_BLOCK - 1) / FX_SCAN2_BLOCK;
+  if (nb > FX_SCAN2_MAX_BLOCKS) {
+    // each block of the second launch reads the totals before it: beyond a
+    // few thousand blocks the three-level scan reads less
+    if (in == m_Accel.gpu(AGRIDCNT))
+      return ScanLevels(in, out, m_Accel.gpu(AAUXARRAY1), m_Accel.gpu(AAUXSCAN1), m_Accel.gpu(AAUXARRAY2),
+                        m_Accel.gpu(AAUXSCAN2), n);
+    return ScanLevels(in, out, m_Accel.gpu(ATAILAUX1), m_Accel.gpu(ATAILAUXS1), m_Accel.gpu(ATAILAUX2),
+                      m_Accel.gpu(ATAILAUXS2), n);
+  }
+  void* a1[] = {&in, &sums, &n};
+  FX_TRY(Launch(FUNC_SCAN_SUMS, (unsigned)nb, FX_SCAN2_THREADS, a1));
+  void* a2[] = {&in, &out, &sums, &n};
+  return Launch(FUNC_SCAN_APPLY, (unsigned)nb, FX_SCAN2_THREADS, a2);
 }
 
 int Particles::ScanLevels(void* array1, void* scan1, void* array2, void* scan2, void* array3, void* scan3,

@@ -48,7 +48,12 @@
 #define FUNC_PRESS_BP 27
 #define FUNC_FORCE_BP 28
 #define FUNC_ADVANCE_BP 29
-#define FUNC_MAX 30
+#define FUNC_SCAN_SUMS 30  // the step's two-launch scans
+#define FX_SCAN2_THREADS 512  // threads per block of fxScanSums / fxScanApply
+#define FX_SCAN2_BLOCK 2048   // elements per block (4 per thread)
+#define FX_SCAN2_MAX_BLOCKS 8192
+#define FUNC_SCAN_APPLY 31
+#define FUNC_MAX 32
 
 #include <map>
 #include <string>
@@ -152,6 +157,9 @@ class Particles {
   // one thread per particle, with the block size chosen for the stage
   int LaunchP(int fid, int stage, void** args);
   int ScanLevels(void* array1, void* scan1, void* array2, void* scan2, void* array3, void* scan3, int n);
+  // exclusive scan in two launches (fxScanSums, fxScanApply); sums holds one
+  // uint per block of 2048 elements
+  int ScanTwoLaunch(void* in, void* out, void* sums, int n);
   int PublishTables();
   void ReleaseGraph();
   int BuildBlockTables();

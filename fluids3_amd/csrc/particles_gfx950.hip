@@ -380,6 +380,72 @@ extern "C" __global__ __launch_bounds__(SCAN_BLOCKSIZE) void prefixFixup(
   if (start + SCAN_BLOCKSIZE < (uint)len) input[start + SCAN_BLOCKSIZE] += add;
 }
 
+// The step's own scans (cell counts -> offsets, out-of-range flags -> tail
+// ranks) in two launches each instead of the three to five of the
+// reference's prefixSum/prefixFixup levels: fxScanSums writes the total of
+// every block of FXS_BLOCK elements; in fxScanApply block b adds up the
+// totals of the blocks before it (a few thousand at most, read once per
+// block) and scans its own elements on top.  Exclusive, uint32: the same
+// results as the reference-named kernels.
+#define FXS_THREADS 512
+#define FXS_PER 4
+#define FXS_BLOCK (FXS_THREADS * FXS_PER)
+
+// elements i..i+3 (zero beyond n); one 16-byte load when all four exist
+__device__ __forceinline__ uint4 fxs_load(const uint* __restrict__ in, int i, int n) {
+  if (i + 3 < n) return *reinterpret_cast<const uint4*>(in + i);
+  uint4 v = make_uint4(0u, 0u, 0u, 0u);
+  if (i < n) v.x = in[i];
+  if (i + 1 < n) v.y = in[i + 1];
+  if (i + 2 < n) v.z = in[i + 2];
+  return v;
+}
+
+// sum of v over the block, returned to every thread (lds: one uint per wave)
+__device__ __forceinline__ uint fxs_block_sum(uint v, uint* lds) {
+  const uint tot = (uint)__builtin_amdgcn_readlane((int)wave_inclusive_scan(v), 63);
+  if ((threadIdx.x & 63u) == 0u) lds[threadIdx.x >> 6] = tot;
+  __syncthreads();
+  uint s = 0u;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += lds[w];
+  __syncthreads();  // every thread has read lds before it is written again
+  return s;
+}
+
+extern "C" __global__ __launch_bounds__(FXS_THREADS) void fxScanSums(const uint* __restrict__ in,
+                                                                      uint* __restrict__ sums, int n) {
+  __shared__ uint lds[FXS_THREADS / 64];
+  const uint4 v = fxs_load(in, (int)blockIdx.x * FXS_BLOCK + (int)threadIdx.x * FXS_PER, n);
+  const uint s = fxs_block_sum(v.x + v.y + v.z + v.w, lds);
+  if (threadIdx.x == 0) sums[blockIdx.x] = s;
+}
+
+extern "C" __global__ __launch_bounds__(FXS_THREADS) void fxScanApply(const uint* __restrict__ in,
+                                                                       uint* __restrict__ out,
+                                                                       const uint* __restrict__ sums, int n) {
+  __shared__ uint lds[FXS_THREADS / 64];
+  uint pre = 0u;
+  for (int j = (int)threadIdx.x; j < (int)blockIdx.x; j += FXS_THREADS) pre += sums[j];
+  pre = fxs_block_sum(pre, lds);
+  const int i = (int)blockIdx.x * FXS_BLOCK + (int)threadIdx.x * FXS_PER;
+  const uint4 v = fxs_load(in, i, n);
+  const uint t = v.x + v.y + v.z + v.w;
+  const uint inc = wave_inclusive_scan(t);
+  const int wave = (int)(threadIdx.x >> 6);
+  if ((threadIdx.x & 63u) == 63u) lds[wave] = inc;
+  __syncthreads();
+  uint wpre = 0u;
+  for (int w = 0; w < wave; w++) wpre += lds[w];
+  const uint e = pre + wpre + inc - t;
+  if (i + 3 < n) {
+    *reinterpret_cast<uint4*>(out + i) = make_uint4(e, e + v.x, e + v.x + v.y, e + v.x + v.y + v.z);
+  } else {
+    if (i < n) out[i] = e;
+    if (i + 1 < n) out[i + 1] = e + v.x;
+    if (i + 2 < n) out[i + 2] = e + v.x + v.y;
+  }
+}
+
 // ------------------------------------------------------------ counting sort
 
 // x slab of an in-range position inside its cell, from the insert's float
