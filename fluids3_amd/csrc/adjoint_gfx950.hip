@@ -212,14 +212,26 @@ extern "C" __global__ void adjTapeRecord(TapeSrc S, char* dst, int n, int cells)
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   const size_t N = (size_t)n;
   if (a < n) {
-    fx_float3 v[4];
+    // as twelve floats and three words (an array of fx_float3 copies would
+    // be placed in LDS by the compiler, 48 KB per block)
+    float v[12];
     uint w[3];
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] = S.v3[k][a];
+    for (int k = 0; k < 4; k++) {
+      const float* f = reinterpret_cast<const float*>(S.v3[k]) + 3 * (size_t)a;
+      v[3 * k] = f[0];
+      v[3 * k + 1] = f[1];
+      v[3 * k + 2] = f[2];
+    }
 #pragma unroll
     for (int k = 0; k < 3; k++) w[k] = S.s1[k][a];
 #pragma unroll
-    for (int k = 0; k < 4; k++) reinterpret_cast<fx_float3*>(dst + 12 * N * k)[a] = v[k];
+    for (int k = 0; k < 4; k++) {
+      float* f = reinterpret_cast<float*>(dst + 12 * N * k) + 3 * (size_t)a;
+      f[0] = v[3 * k];
+      f[1] = v[3 * k + 1];
+      f[2] = v[3 * k + 2];
+    }
 #pragma unroll
     for (int k = 0; k < 3; k++) reinterpret_cast<uint*>(dst + 48 * N + 4 * N * k)[a] = w[k];
   }

@@ -61,6 +61,20 @@ __device__ __forceinline__ T* pbuf(const cuDataX& d, int id) {
 
 // A scene's parameters read through the constant address space, so that the
 // (block-uniform) reads are scalar loads.
+// Three floats of element i, loaded and stored as scalars: an fx_float3
+// copied as a whole (a struct select, say) can leave the compiler an
+// aggregate it places in LDS or scratch memory instead of registers.
+__device__ __forceinline__ fx_float3 ld3(const fx_float3* __restrict__ p, size_t i) {
+  const float* f = reinterpret_cast<const float*>(p) + 3 * i;
+  return fx_float3{f[0], f[1], f[2]};
+}
+__device__ __forceinline__ void st3(fx_float3* __restrict__ p, size_t i, float x, float y, float z) {
+  float* f = reinterpret_cast<float*>(p) + 3 * i;
+  f[0] = x;
+  f[1] = y;
+  f[2] = z;
+}
+
 __device__ __forceinline__ const FParams_t* cparams(const FParams_t* p) {
   typedef const __attribute__((address_space(4))) FParams_t cP;
   return (const FParams_t*)(cP*)p;
@@ -518,8 +532,14 @@ __device__ __forceinline__ void sort_full_impl(const FParams_t& P, const Span S)
   const int cell = (c == FX_GRID_UNDEF) ? S.c0 + P.gridTotal : c;
   const uint o = act ? off[cell] : 0u;
   const uint n = act ? cnt[cell] : 0u;
-  const fx_float3 pos = act ? pbuf<const fx_float3>(FPntTmp, FPOS)[i] : fx_float3{0.0f, 0.0f, 0.0f};
-  const int slab = c == FX_GRID_UNDEF ? 0 : slab_of(P, pos.x);
+  float px = 0.0f, py = 0.0f, pz = 0.0f;
+  if (act) {
+    const fx_float3 q = ld3(pbuf<const fx_float3>(FPntTmp, FPOS), (size_t)i);
+    px = q.x;
+    py = q.y;
+    pz = q.z;
+  }
+  const int slab = c == FX_GRID_UNDEF ? 0 : slab_of(P, px);
   const uint key = ((uint)slab << FX_MEMBER_BITS) | (uint)i;  // this particle's own member entry
   const bool wide = act && c != FX_GRID_UNDEF && n > FX_RANK_WAVE && n <= FX_RANK_CAP;
   int rank = 0;
@@ -585,14 +605,14 @@ __device__ __forceinline__ void sort_full_impl(const FParams_t& P, const Span S)
   // every load before the first store (the two sets are distinct buffers,
   // which the compiler cannot know)
   const bool has_id = FPntTmp.mbuf[FID] != 0 && FPnts.mbuf[FID] != 0;
-  const fx_float3 vel = pbuf<const fx_float3>(FPntTmp, FVEL)[i];
-  const fx_float3 vev = pbuf<const fx_float3>(FPntTmp, FVEVAL)[i];
+  const fx_float3 vel = ld3(pbuf<const fx_float3>(FPntTmp, FVEL), (size_t)i);
+  const fx_float3 vev = ld3(pbuf<const fx_float3>(FPntTmp, FVEVAL), (size_t)i);
   const uint clr = pbuf<const uint>(FPntTmp, FCLR)[i];
   const uint id = has_id ? pbuf<const uint>(FPntTmp, FID)[i] : 0u;
   const uint s = o + (uint)rank;
-  pbuf<fx_float3>(FPnts, FPOS)[s] = pos;
-  pbuf<fx_float3>(FPnts, FVEL)[s] = vel;
-  pbuf<fx_float3>(FPnts, FVEVAL)[s] = vev;
+  st3(pbuf<fx_float3>(FPnts, FPOS), s, px, py, pz);
+  st3(pbuf<fx_float3>(FPnts, FVEL), s, vel.x, vel.y, vel.z);
+  st3(pbuf<fx_float3>(FPnts, FVEVAL), s, vev.x, vev.y, vev.z);
   pbuf<uint>(FPnts, FCLR)[s] = clr;
   if (has_id) pbuf<uint>(FPnts, FID)[s] = id;
   pbuf<int>(FPnts, FGCELL)[s] = c;
@@ -615,17 +635,17 @@ __device__ __forceinline__ void sort_full_gndx(const FParams_t& P, const Span S,
   const int nd = pbuf<const int>(FPntTmp, FGNDX)[i];
   const uint s = pbuf<const uint>(FAccel, AGRIDOFF)[cell] + (uint)nd;
   const bool has_id = FPntTmp.mbuf[FID] != 0 && FPnts.mbuf[FID] != 0;
-  const fx_float3 pos = pbuf<const fx_float3>(FPntTmp, FPOS)[i];
-  const fx_float3 vel = pbuf<const fx_float3>(FPntTmp, FVEL)[i];
-  const fx_float3 vev = pbuf<const fx_float3>(FPntTmp, FVEVAL)[i];
-  const fx_float3 frc = pbuf<const fx_float3>(FPntTmp, FFORCE)[i];
+  const fx_float3 pos = ld3(pbuf<const fx_float3>(FPntTmp, FPOS), (size_t)i);
+  const fx_float3 vel = ld3(pbuf<const fx_float3>(FPntTmp, FVEL), (size_t)i);
+  const fx_float3 vev = ld3(pbuf<const fx_float3>(FPntTmp, FVEVAL), (size_t)i);
+  const fx_float3 frc = ld3(pbuf<const fx_float3>(FPntTmp, FFORCE), (size_t)i);
   const float prs = pbuf<const float>(FPntTmp, FPRESS)[i];
   const uint clr = pbuf<const uint>(FPntTmp, FCLR)[i];
   const uint id = has_id ? pbuf<const uint>(FPntTmp, FID)[i] : 0u;
-  pbuf<fx_float3>(FPnts, FPOS)[s] = pos;
-  pbuf<fx_float3>(FPnts, FVEL)[s] = vel;
-  pbuf<fx_float3>(FPnts, FVEVAL)[s] = vev;
-  pbuf<fx_float3>(FPnts, FFORCE)[s] = frc;
+  st3(pbuf<fx_float3>(FPnts, FPOS), s, pos.x, pos.y, pos.z);
+  st3(pbuf<fx_float3>(FPnts, FVEL), s, vel.x, vel.y, vel.z);
+  st3(pbuf<fx_float3>(FPnts, FVEVAL), s, vev.x, vev.y, vev.z);
+  st3(pbuf<fx_float3>(FPnts, FFORCE), s, frc.x, frc.y, frc.z);
   pbuf<float>(FPnts, FPRESS)[s] = prs;
   pbuf<uint>(FPnts, FCLR)[s] = clr;
   if (has_id) pbuf<uint>(FPnts, FID)[s] = id;
