@@ -268,8 +268,15 @@ extern "C" __global__ void fxMark(int tag) {}
 // from the previous step, so the 64 lanes of a wave fall into few cells;
 // lanes of one cell take a single atomicAdd of their number and their slot
 // is the returned base plus their rank among those lanes.  The atomics of
-// all the wave's cells are issued together (no round trip per cell).
-__device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S, bool tail) {
+// all the wave's cells are issued together (no round trip per cell).  The
+// tail bucket is aggregated per block instead: particles that leave the grid
+// stay out of it (the advance parks them), they are sorted to the end, and
+// in a long run whole blocks hold nothing else (a sixth of the particles by
+// step 200 of the 4M dam break), so one count per wave would put thousands
+// of atomics on that one address.  `tl` is the block's LDS (two sets of 17
+// words, alternating between calls so that one call's reads cannot meet
+// the next call's writes).
+__device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S, bool tail, uint (*tl)[17]) {
   const int i = S.p0 + S.blk * (int)blockDim.x + (int)threadIdx.x;
   bool active = i < S.p1;  // below: particles that take a count slot
   int gs = 0;
@@ -292,11 +299,13 @@ __device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S, bo
     if (tflag != nullptr) tflag[i] = in ? 0u : 1u;
   }
   uint* cnt = pbuf<uint>(FAccel, AGRIDCNT);
+  const int tslot = S.c0 + P.gridTotal;
+  const bool intail = active && gs == tslot;
   // group the wave's lanes by slot (register-only ballots), then every
   // group's first lane adds the group's size with one atomic; all groups'
   // atomics go out together and each lane reads its group's base from the
   // leader lane
-  unsigned long long todo = __ballot(active);
+  unsigned long long todo = __ballot(active && !intail);
   unsigned long long mine = 0ull;
   int leader_of = 0;
   while (todo != 0ull) {  // one pass per distinct slot in the wave
@@ -310,21 +319,41 @@ __device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S, bo
     todo &= ~same;
   }
   uint base = 0;
-  if (active && (int)__lane_id() == leader_of) base = atomicAdd(&cnt[gs], (uint)__popcll(mine));
+  if (active && !intail && (int)__lane_id() == leader_of) base = atomicAdd(&cnt[gs], (uint)__popcll(mine));
   base = (uint)__shfl((int)base, leader_of, 64);
+  if (tail) {  // uniform: the block's tail lanes, one atomic
+    const unsigned long long tm = __ballot(intail);
+    const int wv = (int)(threadIdx.x >> 6), nw = (int)((blockDim.x + 63u) >> 6);
+    uint* L = tl[S.blk & 1];
+    if (__lane_id() == 0) L[wv] = (uint)__popcll(tm);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint tot = 0;
+      for (int w = 0; w < nw; w++) tot += L[w];
+      L[16] = tot != 0u ? atomicAdd(&cnt[tslot], tot) : 0u;
+    }
+    __syncthreads();
+    if (intail) {
+      uint b = L[16];
+      for (int w = 0; w < wv; w++) b += L[w];
+      base = b;
+      mine = tm;
+    }
+  }
   if (active) pbuf<int>(FPnts, FGNDX)[i] = (int)(base + lanes_below(mine));
 }
 
 // The streaming kernels keep a straight-line body for one particle per
 // thread (the reference's launches and the usual choice), the loop otherwise.
 extern "C" __global__ void insertParticles(int pnum) {
+  __shared__ uint tl[2][17];
   const int ppt = ppt_of(0);
   const Span S{0, pnum, 0, (int)blockIdx.x};
   if (ppt == 1) {
-    insert_impl(*cparams(&FParams), S, host_tail());
+    insert_impl(*cparams(&FParams), S, host_tail(), tl);
     return;
   }
-  for (int k = 0; k < ppt; k++) insert_impl(*cparams(&FParams), sub_span(S, ppt, k), host_tail());
+  for (int k = 0; k < ppt; k++) insert_impl(*cparams(&FParams), sub_span(S, ppt, k), host_tail(), tl);
 }
 
 // one block-table entry of a batch stage (the *Batch kernels run one per
@@ -333,7 +362,8 @@ __device__ __forceinline__ void insert_entry(const BatchArgs& B, int b) {
   const int ppt = ppt_of(0);
   const FParams_t* P;
   const Span S = batch_span(B, b, P);
-  for (int k = 0; k < ppt; k++) insert_impl(*P, sub_span(S, ppt, k), true);
+  __shared__ uint tl[2][17];
+  for (int k = 0; k < ppt; k++) insert_impl(*P, sub_span(S, ppt, k), true, tl);
 }
 
 extern "C" __global__ void insertParticlesBatch(BatchArgs B) { insert_entry(B, blockIdx.x); }
