@@ -17,7 +17,7 @@
 static const char* kKernelNames[FUNC_MAX] = {
     "insertParticles",   "countingSortFullRanked", "computePressure", "computeForce",
     "advanceParticles",  "prefixSum",        "prefixFixup",     "countingSortIndex",
-    "fxFillUint",        "advanceParticlesClock", "fxClockTick",     "fxCountPairs",
+    "fxFillUint",        "advanceParticlesClock", "fxPublishTables", "fxCountPairs",
     "insertParticlesBatch", "countingSortIndexBatch", "countingSortFullBatch", "computePressureBatch",
     "computeForceBatch", "advanceParticlesBatch", "fxCountPairsBatch", "sampleParticles",
     "emitParticles",     "fxMaxWords",       "fxMaxWordsBatch",  "fxMark",
@@ -81,7 +81,7 @@ int Particles::Initialize(const char* path) {
   FX_HIP(hipMemcpyHtoD(caps, &capv, sizeof(int)), "Initialize");
   FX_HIP(hipStreamCreate(&m_Stream), "Initialize");
   FX_HIP(hipMalloc(&m_Tables, 2 * sizeof(cuDataX)), "Initialize");
-  FX_HIP(hipMalloc((void**)&m_Clock, sizeof(float)), "Initialize");
+  FX_HIP(hipMalloc((void**)&m_Clock, 2 * sizeof(float)), "Initialize");
   FX_HIP(hipMalloc(&m_Stats, 2 * sizeof(unsigned long long)), "Initialize");
   FX_HIP(hipMalloc((void**)&m_WorkQ, 8 * FX_NUM_STAGES * sizeof(uint32_t)), "Initialize");
   FX_HIP(hipModuleGetGlobal(reinterpret_cast<hipDeviceptr_t*>(&m_PptSym), &len, m_Module, "FPpt"), "Initialize");
@@ -472,27 +472,49 @@ int Particles::PrefixScanParticles() {
   // Two launches per scan (the block totals go to the first auxiliary
   // level, which holds one entry per 1024 elements); the reference's
   // three-level prefixSum/prefixFixup beyond FX_SCAN2_MAX_BLOCKS blocks.
-  FX_TRY(ScanTwoLaunch(m_Accel.gpu(AGRIDCNT), m_Accel.gpu(AGRIDOFF), m_Accel.gpu(AAUXARRAY1), m_ScanCells));
-  FX_TRY(ScanTwoLaunch(m_Accel.gpu(ATAILFLAG), m_Accel.gpu(ATAILSCAN), m_Accel.gpu(ATAILAUX1), mNumPoints));
+  // both scans share the two launches
+  FX_TRY(ScanTwoLaunch(m_Accel.gpu(AGRIDCNT), m_Accel.gpu(AGRIDOFF), m_Accel.gpu(AAUXARRAY1), m_ScanCells,
+                       m_Accel.gpu(ATAILFLAG), m_Accel.gpu(ATAILSCAN), m_Accel.gpu(ATAILAUX1), mNumPoints));
   m_Done = FX_STAGE_PREFIX;
   return FX_OK;
 }
 
-int Particles::ScanTwoLaunch(void* in, void* out, void* sums, int n) {
+namespace {
+// kernel argument of fxScanSums / fxScanApply (same layout as the kernel's)
+struct ScanJob {
+  const void* in;
+  void* out;
+  void* sums;
+  int n;
+  int nb;
+};
+struct ScanJobs {
+  ScanJob j[2];
+};
+}  // namespace
+
+int Particles::ScanTwoLaunch(void* in, void* out, void* sums, int n, void* in2, void* out2, void* sums2, int n2) {
   const int nb = (n + FX_SCAN2_BLOCK - 1) / FX_SCAN2_BLOCK;
-  if (nb > FX_SCAN2_MAX_BLOCKS) {
+  const int nb2 = (n2 + FX_SCAN2_BLOCK - 1) / FX_SCAN2_BLOCK;
+  if (nb > FX_SCAN2_MAX_BLOCKS || nb2 > FX_SCAN2_MAX_BLOCKS) {
     // each block of the second launch reads the totals before it: beyond a
     // few thousand blocks the three-level scan reads less
-    if (in == m_Accel.gpu(AGRIDCNT))
-      return ScanLevels(in, out, m_Accel.gpu(AAUXARRAY1), m_Accel.gpu(AAUXSCAN1), m_Accel.gpu(AAUXARRAY2),
-                        m_Accel.gpu(AAUXSCAN2), n);
-    return ScanLevels(in, out, m_Accel.gpu(ATAILAUX1), m_Accel.gpu(ATAILAUXS1), m_Accel.gpu(ATAILAUX2),
-                      m_Accel.gpu(ATAILAUXS2), n);
+    auto one = [&](void* a, void* o, void* s, int m) -> int {
+      if ((m + FX_SCAN2_BLOCK - 1) / FX_SCAN2_BLOCK <= FX_SCAN2_MAX_BLOCKS) return ScanTwoLaunch(a, o, s, m);
+      if (a == m_Accel.gpu(AGRIDCNT))
+        return ScanLevels(a, o, m_Accel.gpu(AAUXARRAY1), m_Accel.gpu(AAUXSCAN1), m_Accel.gpu(AAUXARRAY2),
+                          m_Accel.gpu(AAUXSCAN2), m);
+      return ScanLevels(a, o, m_Accel.gpu(ATAILAUX1), m_Accel.gpu(ATAILAUXS1), m_Accel.gpu(ATAILAUX2),
+                        m_Accel.gpu(ATAILAUXS2), m);
+    };
+    FX_TRY(one(in, out, sums, n));
+    return n2 > 0 ? one(in2, out2, sums2, n2) : FX_OK;
   }
-  void* a1[] = {&in, &sums, &n};
-  FX_TRY(Launch(FUNC_SCAN_SUMS, (unsigned)nb, FX_SCAN2_THREADS, a1));
-  void* a2[] = {&in, &out, &sums, &n};
-  return Launch(FUNC_SCAN_APPLY, (unsigned)nb, FX_SCAN2_THREADS, a2);
+  ScanJobs J = {{{in, out, sums, n, nb}, {in2, out2, sums2, n2, nb2}}};
+  if (nb + nb2 == 0) return FX_OK;
+  void* a[] = {&J};
+  FX_TRY(Launch(FUNC_SCAN_SUMS, (unsigned)(nb + nb2), FX_SCAN2_THREADS, a));
+  return Launch(FUNC_SCAN_APPLY, (unsigned)(nb + nb2), FX_SCAN2_THREADS, a);
 }
 
 int Particles::ScanLevels(void* array1, void* scan1, void* array2, void* scan2, void* array3, void* scan3,
@@ -519,12 +541,22 @@ int Particles::ScanLevels(void* array1, void* scan1, void* array2, void* scan2, 
 }
 
 int Particles::PublishTables() {
-  // FPnts <- table of the set m_Points now holds, FPntTmp <- the other one
-  char* tab = static_cast<char*>(m_Tables);
-  FX_HIP(hipMemcpyAsync(m_Points.SymbolAddress(), tab + m_Parity * sizeof(cuDataX), sizeof(cuDataX),
-                        hipMemcpyDeviceToDevice, m_Stream), "PublishTables");
-  FX_HIP(hipMemcpyAsync(m_PointsTemp.SymbolAddress(), tab + (m_Parity ^ 1) * sizeof(cuDataX),
-                        sizeof(cuDataX), hipMemcpyDeviceToDevice, m_Stream), "PublishTables");
+  // FPnts <- table of the set m_Points now holds, FPntTmp <- the other one;
+  // in a captured step the same launch moves the step clock (the advance of
+  // this step reads clock[0])
+  void* bank = m_Tables;
+  void* pnts = m_Points.SymbolAddress();
+  void* tmp = m_PointsTemp.SymbolAddress();
+  void* clk = m_Capturing ? (void*)m_Clock : nullptr;
+  float dt = m_Params.dt;
+  int parity = m_Parity;
+  void* args[] = {&bank, &parity, &pnts, &tmp, &clk, &dt};
+  return Launch(FUNC_PUBLISH, 1, 64, args);
+}
+
+int Particles::SetDeviceClock() {
+  // clock[1] is the time of the next step a captured graph runs
+  FX_HIP(hipMemcpyHtoD(m_Clock + 1, &m_Params.time, sizeof(float)), "device clock");
   return FX_OK;
 }
 
@@ -572,19 +604,16 @@ int Particles::Advance() {
   // particles.cpp:413-418 (launched over pnum, which equals mNumPoints here)
   float dt = m_Params.dt, ss = m_Params.sim_scale;
   int n = m_Params.pnum;
+  // a captured step reads its time from the device clock, which the sort
+  // stage's table exchange (PublishTables) has set
   if (m_Batch) {
     float time = m_Params.time;
     void* clk = m_Capturing ? (void*)m_Clock : nullptr;
-    FX_TRY(BatchLaunch(FUNC_ADVANCE_B, FUNC_ADVANCE_BP, FX_STAGE_ADVANCE, &clk, &time, nullptr));
-    if (!m_Capturing) return FX_OK;
-    void* targs[] = {&m_Clock, &dt};
-    return Launch(FUNC_CLOCK_TICK, 1, 64, targs);
+    return BatchLaunch(FUNC_ADVANCE_B, FUNC_ADVANCE_BP, FX_STAGE_ADVANCE, &clk, &time, nullptr);
   }
   if (m_Capturing) {
     void* args[] = {&m_Clock, &dt, &ss, &n};
-    FX_TRY(LaunchP(FUNC_ADVANCE_CLOCK, FX_STAGE_ADVANCE, args));
-    void* targs[] = {&m_Clock, &dt};
-    return Launch(FUNC_CLOCK_TICK, 1, 64, targs);
+    return LaunchP(FUNC_ADVANCE_CLOCK, FX_STAGE_ADVANCE, args);
   }
   float time = m_Params.time;
   void* args[] = {&time, &dt, &ss, &n};
@@ -689,7 +718,7 @@ int Particles::LaunchGraph(int nlaunch) {
   if (m_GraphExec == nullptr || m_GraphParity != m_Parity) FX_TRY(CaptureGraph());
   // the device clock starts from the host time of the first replayed step
   FX_HIP(hipStreamSynchronize(m_Stream), "LaunchGraph");
-  FX_HIP(hipMemcpyHtoD(m_Clock, &m_Params.time, sizeof(float)), "LaunchGraph");
+  FX_TRY(SetDeviceClock());
   for (int k = 0; k < nlaunch; k++) {
     FX_HIP(hipGraphLaunch(m_GraphExec, m_Stream), "LaunchGraph");
     AdvanceTime();
@@ -954,8 +983,7 @@ int Particles::BenchSteps(int nsteps, bool use_graph, double* ms_total, double* 
       if (st == FX_OK && e != hipSuccess) st = fx::hip_fail(e, "bench", "hipStreamEndCapture");
       if (st == FX_OK && hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
         st = fx::fail(FX_EHIP, "bench: instantiate");
-      if (st == FX_OK && hipMemcpyHtoD(m_Clock, &m_Params.time, sizeof(float)) != hipSuccess)
-        st = fx::fail(FX_EHIP, "bench: clock");
+      if (st == FX_OK) st = SetDeviceClock();
       if (st == FX_OK && hipGraphLaunch(ge, m_Stream) != hipSuccess) st = fx::fail(FX_EHIP, "bench: launch");
       if (st == FX_OK)
         for (int s = 0; s < nsteps; s++) AdvanceTime();
@@ -1006,7 +1034,7 @@ int Particles::TimeSteps(int nsteps, bool use_graph, double* ms_total) {
   if (use_graph) {
     // the device clock must hold the host time before the timed region
     FX_HIP(hipStreamSynchronize(m_Stream), "time_steps");
-    FX_HIP(hipMemcpyHtoD(m_Clock, &m_Params.time, sizeof(float)), "time_steps");
+    FX_TRY(SetDeviceClock());
   }
   fx::Event ea, eb;
   FX_HIP(ea.create(), "time_steps");

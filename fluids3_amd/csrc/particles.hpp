@@ -28,7 +28,7 @@
 #define FUNC_SORT_INDEX 7
 #define FUNC_FILL 8
 #define FUNC_ADVANCE_CLOCK 9
-#define FUNC_CLOCK_TICK 10
+#define FUNC_PUBLISH 10  // fxPublishTables: the set exchange (and the step clock of a captured step)
 #define FUNC_COUNT_PAIRS 11
 #define FUNC_INSERT_B 12
 #define FUNC_SORT_INDEX_B 13
@@ -159,8 +159,11 @@ class Particles {
   int ScanLevels(void* array1, void* scan1, void* array2, void* scan2, void* array3, void* scan3, int n);
   // exclusive scan in two launches (fxScanSums, fxScanApply); sums holds one
   // uint per block of 2048 elements
-  int ScanTwoLaunch(void* in, void* out, void* sums, int n);
+  // exclusive scans of up to two arrays (n2 = 0: one) in two launches
+  int ScanTwoLaunch(void* in, void* out, void* sums, int n, void* in2 = nullptr, void* out2 = nullptr,
+                    void* sums2 = nullptr, int n2 = 0);
   int PublishTables();
+  int SetDeviceClock();
   void ReleaseGraph();
   int BuildBlockTables();
   // one batch launch of a stage: kernel fid, one block per block-table
@@ -199,7 +202,7 @@ class Particles {
   // the last stage (FX_STAGE_*) run on the device since the last insert, -1
   // after a restart; Stage() refuses a stage whose input tables it has not made
   int m_Done = -1;
-  float* m_Clock = nullptr;
+  float* m_Clock = nullptr;  // device {time of the step being run, time of the next step}
   void* m_Stats = nullptr;  // two uint64 counters of fxCountPairs
   void* m_ScanScratch = nullptr;  // this device's scratch array (noalloc.hip), not owned
   void* m_NbrWordsSym = nullptr;  // the module's FNbrWords (bitmask words a particle may record)

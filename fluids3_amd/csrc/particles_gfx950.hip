@@ -146,6 +146,15 @@ __device__ __forceinline__ int wave_reduce(int v) {
 __device__ __forceinline__ int wave_max(int v) { return wave_reduce<true>(v); }
 __device__ __forceinline__ int wave_min(int v) { return wave_reduce<false>(v); }
 
+// Block barrier that orders LDS accesses only: global loads and atomics
+// already in flight stay in flight across it (a __syncthreads() would first
+// wait for them).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ uint lanes_below(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi((uint)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint)mask, 0u));
 }
@@ -318,27 +327,30 @@ __device__ __forceinline__ void insert_impl(const FParams_t& P, const Span S, bo
     }
     todo &= ~same;
   }
-  uint base = 0;
+  uint base = 0, tbase = 0;
   if (active && !intail && (int)__lane_id() == leader_of) base = atomicAdd(&cnt[gs], (uint)__popcll(mine));
-  base = (uint)__shfl((int)base, leader_of, 64);
-  if (tail) {  // uniform: the block's tail lanes, one atomic
-    const unsigned long long tm = __ballot(intail);
+  unsigned long long tm = 0ull;
+  if (tail) {  // uniform: the block's tail lanes, one atomic (the cell atomics stay in flight meanwhile)
+    tm = __ballot(intail);
     const int wv = (int)(threadIdx.x >> 6), nw = (int)((blockDim.x + 63u) >> 6);
     uint* L = tl[S.blk & 1];
     if (__lane_id() == 0) L[wv] = (uint)__popcll(tm);
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
       uint tot = 0;
       for (int w = 0; w < nw; w++) tot += L[w];
       L[16] = tot != 0u ? atomicAdd(&cnt[tslot], tot) : 0u;
     }
-    __syncthreads();
+    lds_barrier();
     if (intail) {
-      uint b = L[16];
-      for (int w = 0; w < wv; w++) b += L[w];
-      base = b;
-      mine = tm;
+      tbase = L[16];
+      for (int w = 0; w < wv; w++) tbase += L[w];
     }
+  }
+  base = (uint)__shfl((int)base, leader_of, 64);
+  if (intail) {
+    base = tbase;
+    mine = tm;
   }
   if (active) pbuf<int>(FPnts, FGNDX)[i] = (int)(base + lanes_below(mine));
 }
@@ -456,22 +468,47 @@ __device__ __forceinline__ uint fxs_block_sum(uint v, uint* lds) {
   return s;
 }
 
-extern "C" __global__ __launch_bounds__(FXS_THREADS) void fxScanSums(const uint* __restrict__ in,
-                                                                      uint* __restrict__ sums, int n) {
-  __shared__ uint lds[FXS_THREADS / 64];
-  const uint4 v = fxs_load(in, (int)blockIdx.x * FXS_BLOCK + (int)threadIdx.x * FXS_PER, n);
-  const uint s = fxs_block_sum(v.x + v.y + v.z + v.w, lds);
-  if (threadIdx.x == 0) sums[blockIdx.x] = s;
+// One launch serves up to two independent scans (the step scans the cell
+// counts and the out-of-range flags): blocks [0, j[0].nb) take the first,
+// the rest the second.
+struct FxScanJob {
+  const uint* in;
+  uint* out;
+  uint* sums;  // one total per block of FXS_BLOCK elements
+  int n;
+  int nb;  // blocks: ceil(n / FXS_BLOCK)
+};
+struct FxScanJobs {
+  FxScanJob j[2];
+};
+
+__device__ __forceinline__ const FxScanJob& fxs_job(const FxScanJobs& J, int& b) {
+  b = (int)blockIdx.x;
+  if (b < J.j[0].nb) return J.j[0];
+  b -= J.j[0].nb;
+  return J.j[1];
 }
 
-extern "C" __global__ __launch_bounds__(FXS_THREADS) void fxScanApply(const uint* __restrict__ in,
-                                                                       uint* __restrict__ out,
-                                                                       const uint* __restrict__ sums, int n) {
+extern "C" __global__ __launch_bounds__(FXS_THREADS) void fxScanSums(FxScanJobs J) {
   __shared__ uint lds[FXS_THREADS / 64];
+  int b;
+  const FxScanJob& S = fxs_job(J, b);
+  const uint4 v = fxs_load(S.in, b * FXS_BLOCK + (int)threadIdx.x * FXS_PER, S.n);
+  const uint s = fxs_block_sum(v.x + v.y + v.z + v.w, lds);
+  if (threadIdx.x == 0) S.sums[b] = s;
+}
+
+extern "C" __global__ __launch_bounds__(FXS_THREADS) void fxScanApply(FxScanJobs J) {
+  __shared__ uint lds[FXS_THREADS / 64];
+  int b;
+  const FxScanJob& S = fxs_job(J, b);
+  const uint* __restrict__ in = S.in;
+  uint* __restrict__ out = S.out;
+  const int n = S.n;
   uint pre = 0u;
-  for (int j = (int)threadIdx.x; j < (int)blockIdx.x; j += FXS_THREADS) pre += sums[j];
+  for (int j = (int)threadIdx.x; j < b; j += FXS_THREADS) pre += S.sums[j];
   pre = fxs_block_sum(pre, lds);
-  const int i = (int)blockIdx.x * FXS_BLOCK + (int)threadIdx.x * FXS_PER;
+  const int i = b * FXS_BLOCK + (int)threadIdx.x * FXS_PER;
   const uint4 v = fxs_load(in, i, n);
   const uint t = v.x + v.y + v.z + v.w;
   const uint inc = wave_inclusive_scan(t);
@@ -1797,8 +1834,8 @@ extern "C" __global__ void advanceParticles(float time, float dt, float ss, int 
 
 // Same integrator with the simulation time read from device memory, for
 // captured step graphs whose launches cannot carry a per-step time argument.
-// fxClockTick advances that clock by dt (the host's AdvanceTime,
-// particles.cpp:396-400).
+// fxPublishTables moves that clock on by dt each step (the host's
+// AdvanceTime, particles.cpp:396-400).
 extern "C" __global__ void advanceParticlesClock(const float* clock, float dt, float ss, int numPnts) {
   const int ppt = ppt_of(5);
   if (ppt == 1) {
@@ -1812,8 +1849,23 @@ extern "C" __global__ void advanceParticlesClock(const float* clock, float dt, f
   }
 }
 
-extern "C" __global__ void fxClockTick(float* clock, float dt) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) clock[0] = clock[0] + dt;
+// The set exchange of the counting sort (one launch, 64 threads): FPnts and
+// FPntTmp receive the two 128-byte tables of bank[parity] and
+// bank[parity ^ 1].  In a captured step it also moves the step clock: clock[1]
+// holds the time of the step about to run, which becomes clock[0] (read by
+// the advance of this step) while clock[1] moves on by dt.
+extern "C" __global__ void fxPublishTables(const uint* __restrict__ bank, int parity, uint* __restrict__ pnts,
+                                           uint* __restrict__ tmp, float* __restrict__ clock, float dt) {
+  const int t = (int)threadIdx.x;
+  constexpr int W = (int)(sizeof(cuDataX) / sizeof(uint));
+  static_assert(2 * W <= 64, "one thread per word of the two tables");
+  if (t < W) pnts[t] = bank[parity * W + t];
+  else if (t < 2 * W) tmp[t - W] = bank[(parity ^ 1) * W + t - W];
+  if (clock != nullptr && t == 0) {
+    const float now = clock[1];
+    clock[0] = now;
+    clock[1] = now + dt;
+  }
 }
 
 // Batch integrator: the time comes from clock[0] when clock is given, else
