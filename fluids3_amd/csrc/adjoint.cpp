@@ -67,6 +67,9 @@ struct AdjArgs {
   void* gx;
   void* gv;
   void* ge;
+  void* gxp;
+  void* gvp;
+  void* gep;
   void* gF;
   void* gxs;
   void* ges;
@@ -82,7 +85,8 @@ struct AdjArgs {
 };
 
 const size_t kCkptBytes = 44;   // pos, vel, veval (12 B each), clr, id (4 B each)
-const size_t kTapeBytes = 60;   // pos, vel, veval, force (12 B each), rho, id, gcell (4 B each)
+const size_t kTapeBytes = 60;   // pos, vel, veval, force (12 B each), rho, sort permutation, gcell (4 B each)
+const size_t kVecFloats = 34;   // floats per particle of m_AdjVec
 // plus, per recorded step, the density kernel's neighbour bitmask
 // (m_AdjWords words per particle), written there directly by the forward
 
@@ -105,6 +109,8 @@ void Particles::AdjRelease() {
   m_AdjArec = m_AdjDrec = m_AdjTabs = nullptr;
   m_AdjVec = nullptr;
   m_AdjId0 = nullptr;
+  m_AdjSrc = nullptr;
+  m_AdjSrcPtr = nullptr;
   m_AdjPartial = m_AdjAccum = nullptr;
   m_AdjParams = m_AdjPBase = m_AdjBTab = m_AdjBFirst = nullptr;
   m_AdjBytes = 0;
@@ -156,7 +162,7 @@ int Particles::AdjConfigure(const char* path, int nsteps, int every) {
   size_t free_b = 0, total_b = 0;
   FX_HIP(hipMemGetInfo(&free_b, &total_b), "adjoint: querying device memory");
   const double budget = 0.6 * (double)free_b;
-  const double fixed = (double)(sizeof(float) * 25 + 4 + 48 + 16) * n;
+  const double fixed = (double)(sizeof(float) * kVecFloats + 4 + 4 + 48 + 16) * n;
   const double per_step = (double)tape_step + 4.0 * words * n;
   if (every <= 0) {
     // The backward recomputes every segment but the last, (segments - 1) x
@@ -199,10 +205,18 @@ int Particles::AdjConfigure(const char* path, int nsteps, int every) {
   m_AdjCkpt = static_cast<char*>(p);
   FX_TRY(alloc(tape_step * (size_t)tape_steps, &p));
   m_AdjTape = static_cast<char*>(p);
-  FX_TRY(alloc(sizeof(float) * (size_t)n * 25, &p));
+  FX_TRY(alloc(sizeof(float) * (size_t)n * kVecFloats, &p));
   m_AdjVec = static_cast<float*>(p);
   FX_TRY(alloc(sizeof(uint32_t) * (size_t)n, &p));
   m_AdjId0 = static_cast<uint32_t*>(p);
+  FX_TRY(alloc(sizeof(uint32_t) * (size_t)n, &p));
+  m_AdjSrc = static_cast<uint32_t*>(p);
+  FX_TRY(alloc(2 * sizeof(void*), &p));
+  m_AdjSrcPtr = static_cast<void**>(p);
+  {
+    void* ptrs[2] = {m_AdjSrc, nullptr};
+    FX_HIP(hipMemcpyHtoD(m_AdjSrcPtr, ptrs, sizeof(ptrs)), "adjoint");
+  }
   FX_TRY(alloc(sizeof(uint32_t) * (size_t)words * (size_t)n * (size_t)tape_steps, &p));
   m_AdjBits = static_cast<char*>(p);
   FX_TRY(alloc(2 * sizeof(int), &p));
@@ -336,7 +350,7 @@ int Particles::AdjStep(int record_slot) {
     const size_t n = (size_t)m_AdjN;
     char* t = m_AdjTape + (kTapeBytes * n + 8 * (size_t)m_AdjCells) * (size_t)record_slot;
     TapeSrc src = {{m_Points.gpu(FPOS), m_Points.gpu(FVEL), m_Points.gpu(FVEVAL), m_Points.gpu(FFORCE)},
-                   {m_Points.gpu(FPRESS), m_Points.gpu(FID), m_Points.gpu(FGCELL)},
+                   {m_Points.gpu(FPRESS), m_AdjSrc, m_Points.gpu(FGCELL)},
                    m_Accel.gpu(AGRIDCNT),
                    m_Accel.gpu(AGRIDOFF)};
     int nn = m_AdjN, cells = m_AdjCells;
@@ -346,6 +360,13 @@ int Particles::AdjStep(int record_slot) {
   }
   FX_TRY(Advance());
   AdvanceTime();
+  return FX_OK;
+}
+
+int Particles::AdjSortSrc(bool on) {
+  // the sorts of a rollout write their permutation into m_AdjSrc
+  FX_HIP(hipMemcpyAsync(m_SortSrcSym, m_AdjSrcPtr + (on ? 0 : 1), sizeof(void*), hipMemcpyDeviceToDevice, m_Stream),
+         "adjoint");
   return FX_OK;
 }
 
@@ -367,11 +388,13 @@ int Particles::AdjForward(double* loss, double* ms) {
     FX_TRY(launch_n(m_AdjFunc[ADJ_FRAME_IN], kAdjNames[ADJ_FRAME_IN], m_Stream, nn, a));
   }
   const int last = m_AdjSegs - 1;
+  FX_TRY(AdjSortSrc(true));
   for (int t = 0; t < m_AdjSteps; t++) {
     const int seg = t / m_AdjEvery;
     if (t % m_AdjEvery == 0) FX_TRY(AdjSaveCheckpoint(seg));
     FX_TRY(AdjStep(seg == last ? (seg % m_AdjBufs) * m_AdjTapeSteps + t - seg * m_AdjEvery : -1));
   }
+  FX_TRY(AdjSortSrc(false));
   FX_TRY(AdjSaveCheckpoint(m_AdjSegs));
   // loss and seed on the final state
   const size_t n = (size_t)m_AdjN;
@@ -456,6 +479,11 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
     FX_HIP(hipEventRecord(m_AdjEv[b], m_Stream), "adjoint");
   }
   bool first = true;
+  FX_TRY(AdjSortSrc(true));
+  // the adjoint state alternates between two sets: [cur] holds that of the
+  // step's outputs, the step writes that of its inputs into [cur ^ 1]
+  float* set[2] = {V, V + 25 * n};
+  int cur = 0;
   if (m_AdjSegs > 0 && !m_AdjTapeValid) FX_TRY(AdjRecompute(m_AdjSegs - 1));
   for (int seg = m_AdjSegs - 1; seg >= 0; seg--) {
     if (seg > 0 && m_AdjBufs == 2) FX_TRY(AdjRecompute(seg - 1));  // beside the adjoint of this segment
@@ -480,9 +508,12 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
       A.cnt = T + 60 * n;
       A.off = T + 60 * n + 4 * (size_t)m_AdjCells;
       A.target = V + 22 * n;
-      A.gx = V;
-      A.gv = V + 3 * n;
-      A.ge = V + 6 * n;
+      A.gx = set[cur];
+      A.gv = set[cur] + 3 * n;
+      A.ge = set[cur] + 6 * n;
+      A.gxp = set[cur ^ 1];
+      A.gvp = set[cur ^ 1] + 3 * n;
+      A.gep = set[cur ^ 1] + 6 * n;
       A.gF = V + 9 * n;
       A.gxs = V + 12 * n;
       A.ges = V + 15 * n;
@@ -499,12 +530,14 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
       FX_TRY(AdjLaunch(ADJ_FORCE, &A, m_AdjStream));
       FX_TRY(AdjLaunch(ADJ_DENSITY, &A, m_AdjStream));
       first = false;
+      cur ^= 1;
     }
     FX_HIP(hipEventRecord(m_AdjEv[2 + b], m_AdjStream), "adjoint");
     if (seg > 0 && m_AdjBufs == 1) FX_TRY(AdjRecompute(seg - 1));  // one buffer: after its adjoint
   }
   FX_HIP(hipEventRecord(m_AdjEv[4], m_AdjStream), "adjoint");
   FX_HIP(hipStreamWaitEvent(m_Stream, m_AdjEv[4], 0), "adjoint");
+  FX_TRY(AdjSortSrc(false));
   // the system continues from the end of the rollout, with its original ids;
   // the gradients go back to original-id order (into the gF..ges scratch)
   FX_TRY(AdjRestoreCheckpoint(m_AdjSegs));
@@ -514,7 +547,9 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
     int nn = m_AdjN;
     void* a[] = {&fid, &m_AdjId0, &nn};
     FX_TRY(launch_n(m_AdjFunc[ADJ_FRAME_OUT], kAdjNames[ADJ_FRAME_OUT], m_Stream, nn, a));
-    void* b[] = {&V, &out, &m_AdjId0, &nn};
+    // the adjoint of the rollout's inputs, in its first step's order before the sort
+    float* g = set[cur];
+    void* b[] = {&g, &out, &m_AdjId0, &nn};
     FX_TRY(launch_n(m_AdjFunc[ADJ_UNFRAME], kAdjNames[ADJ_UNFRAME], m_Stream, nn, b));
   }
   m_AdjTapeValid = m_AdjSegs <= 1;

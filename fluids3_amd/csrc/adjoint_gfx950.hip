@@ -8,11 +8,15 @@
 // cells, branch masks and the sort permutation are taken from that forward
 // step as piecewise-constant data.
 //
-// Adjoint variables of the state live in the particle-id frame (index =
-// original particle index): reading and writing them through the recorded
-// ids undoes each step's sort.  Per step, for the step's sorted slots:
+// Adjoint variables of the state live in the slot order of the state they
+// belong to: those of step t's outputs in step t's sorted order, those of
+// its inputs in the order before its sort, which the step's recorded sort
+// permutation src (slot s holds the particle that was at src[s]) gives.
+// Each step reads one set and writes the other, and the two alternate.
+// Per step, for the step's sorted slots:
 //   adjAdvance  (gx', gv', ge') of the outputs -> gF, partial gx/ge of the
-//               inputs (sorted frame), gv of the velocity input (id frame);
+//               inputs (sorted frame), gv of the velocity input (written at
+//               src[s], the order before the sort);
 //   adjForce    gather over in-radius neighbours j (dl = gF_i - gF_j,
 //               u = x_i - x_j, dve = ve_j - ve_i, inv = 1/(r rho_i rho_j)):
 //                 gx_i   += A dl + (d2/r) u (A_r (u.dl) + B_r (dve.dl))
@@ -20,7 +24,7 @@
 //                 grho_i += kp c^2 inv k (u.dl) - (A (u.dl) + B (dve.dl)) / rho_i
 //               and the per-particle terms of dL/dk and dL/dvisc;
 //   adjDensity  gx_i += sum_j (grho_i + grho_j) (-6 m poly6 d2) t^2 u, then
-//               the step's input adjoints go back to the id frame.
+//               the step's input adjoints go to src[s].
 // Every pair term is gathered from i's side (A and B are symmetric in i, j),
 // so there are no atomics and results are reproducible bit for bit.  FP64 is
 // used only for the loss and the two scalar-parameter sums: per-block FP64
@@ -65,15 +69,19 @@ struct AdjArgs {
   const fx_float3* veval;
   const fx_float3* force;
   const float* rho;
-  const uint* id;
+  const uint* id;  // adjLoss: the final state's ids; the step kernels: the step's sort permutation src
   const int* gcell;
   const uint* cnt;
   const uint* off;
-  const fx_float3* target;  // id frame (loss only)
-  // adjoint state (id frame)
+  const fx_float3* target;  // by id (loss only)
+  // adjoint state of the step's outputs (its sorted order) ...
   fx_float3* gx;
   fx_float3* gv;
   fx_float3* ge;
+  // ... and of its inputs (the order before its sort)
+  fx_float3* gxp;
+  fx_float3* gvp;
+  fx_float3* gep;
   // per-step work (sorted frame)
   fx_float3* gF;
   fx_float3* gxs;
@@ -147,8 +155,8 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 }
 
 // ------------------------------------------------------------------- loss
-// L_s = (1/n_s) sum |x - x*|^2 over scene s (FP64), seed gx[id] = (2/n_s)(x - x*),
-// gv = ge = 0.  pos/id: the final state in its slot order.
+// L_s = (1/n_s) sum |x - x*|^2 over scene s (FP64), seed gx = (2/n_s)(x - x*),
+// gv = ge = 0, in the final state's slot order.  pos/id: the final state.
 extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjLoss(AdjArgs A) {
   __shared__ double lds[1];
   const Slot S = slot_of(A);
@@ -159,9 +167,9 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjLoss(AdjArgs A) {
     const float scale = 2.0f / (float)(S.p1 - S.p0);
     const float dx = x.x - t.x, dy = x.y - t.y, dz = x.z - t.z;
     l = (double)dx * (double)dx + (double)dy * (double)dy + (double)dz * (double)dz;
-    A.gx[id] = fx_float3{dx * scale, dy * scale, dz * scale};
-    A.gv[id] = fx_float3{0.0f, 0.0f, 0.0f};
-    A.ge[id] = fx_float3{0.0f, 0.0f, 0.0f};
+    A.gx[S.i] = fx_float3{dx * scale, dy * scale, dz * scale};
+    A.gv[S.i] = fx_float3{0.0f, 0.0f, 0.0f};
+    A.ge[S.i] = fx_float3{0.0f, 0.0f, 0.0f};
   }
   l = block_sum<ADJ_THREADS>(l, lds);
   if (threadIdx.x == 0) A.partial[3 * S.blk] = l;
@@ -169,12 +177,12 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjLoss(AdjArgs A) {
 
 // ------------------------------------------------------------ rollout frame
 // For the length of a rollout the particles are numbered by their slot at its
-// start (FID := slot), so the adjoint variables, indexed by FID, sit close to
-// each step's slot order (particles move little between steps) instead of in
-// the arbitrary order of the original ids.  adjFrameIn saves the original ids,
-// renumbers and moves the target into the frame; adjFrameOut restores the
-// ids of the current state; adjUnframe moves the three gradients back to
-// original-id order.
+// start (FID := slot): the adjoint of the rollout's first step comes out in
+// that order, and the loss reads the target through ids that stay close to
+// the final state's slot order (particles move little between steps).
+// adjFrameIn saves the original ids, renumbers and moves the target into the
+// frame; adjFrameOut restores the ids of the current state; adjUnframe moves
+// the three gradients back to original-id order.
 extern "C" __global__ void adjFrameIn(uint* fid, uint* id0, const fx_float3* target, fx_float3* target_f, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
@@ -279,13 +287,13 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) 
   if (S.i >= S.p1) return;
   const FParams_t& P = *S.P;
   const int s = S.i;
-  const uint id = A.id[s];
-  const fx_float3 gxo = A.gx[id], gvo = A.gv[id], geo = A.ge[id];
+  const uint src = A.id[s];
+  const fx_float3 gxo = A.gx[s], gvo = A.gv[s], geo = A.ge[s];
   if (A.gcell[s] == FX_GRID_UNDEF) {  // parked: pos and vel become constants, veval passes through
     const fx_float3 z = {0.0f, 0.0f, 0.0f};
     A.gxs[s] = z;
     A.ges[s] = geo;
-    A.gv[id] = z;
+    A.gvp[src] = z;
     return;
   }
   const fx_float3 pos = A.pos[s], vel = A.vel[s], ve = A.veval[s], f = A.force[s];
@@ -371,7 +379,7 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjAdvance(AdjArgs A) 
     gay = (gay - ay * d) * sa;
     gaz = (gaz - az * d) * sa;
   }
-  A.gv[id] = fx_float3{gvx, gvy, gvz};
+  A.gvp[src] = fx_float3{gvx, gvy, gvz};
   A.arec[3 * (size_t)s] = make_float4(pos.x, pos.y, pos.z, A.rho[s]);
   A.arec[3 * (size_t)s + 1] = make_float4(ve.x, ve.y, ve.z, 0.0f);
   A.arec[3 * (size_t)s + 2] = make_float4(gax * m, gay * m, gaz * m, 0.0f);
@@ -831,10 +839,10 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjDensity(AdjArgs A) 
   }
   pair_walk<AW_QD>(P, A, i, base, act && !ovf, fq, D);
   if (i >= S.p1) return;
-  const uint id = A.id[i];
+  const uint src = A.id[i];
   const fx_float3 x0 = A.gxs[i];
-  A.gx[id] = fx_float3{x0.x + D.gpx, x0.y + D.gpy, x0.z + D.gpz};
-  A.ge[id] = A.ges[i];
+  A.gxp[src] = fx_float3{x0.x + D.gpx, x0.y + D.gpy, x0.z + D.gpz};
+  A.gep[src] = A.ges[i];
 }
 
 // ----------------------------------------------------------------- reduce

@@ -72,6 +72,9 @@ int Particles::Initialize(const char* path) {
                     sizeof(FParams_t));
   FX_HIP(hipModuleGetGlobal(reinterpret_cast<hipDeviceptr_t*>(&m_NbrWordsSym), &len, m_Module, "FNbrWords"),
          "Initialize");
+  FX_HIP(hipModuleGetGlobal(reinterpret_cast<hipDeviceptr_t*>(&m_SortSrcSym), &len, m_Module, "FSortSrc"),
+         "Initialize");
+  if (len != sizeof(void*)) return fx::fail(FX_EMISMATCH, "Initialize: FSortSrc has %zu bytes", len);
   // this host allocates the tail slot and runs the index sort (a host that
   // binds the module as the reference does leaves FHostCaps at 0)
   hipDeviceptr_t caps = nullptr;

@@ -182,6 +182,7 @@ class Particles {
   int AdjRestoreCheckpoint(int k);
   int AdjLaunch(int fid, void* args, hipStream_t st);
   int AdjRecompute(int seg);
+  int AdjSortSrc(bool on);
 
   hipModule_t m_Module = nullptr;
   hipFunction_t m_Func[FUNC_MAX] = {};
@@ -206,6 +207,7 @@ class Particles {
   void* m_Stats = nullptr;  // two uint64 counters of fxCountPairs
   void* m_ScanScratch = nullptr;  // this device's scratch array (noalloc.hip), not owned
   void* m_NbrWordsSym = nullptr;  // the module's FNbrWords (bitmask words a particle may record)
+  void* m_SortSrcSym = nullptr;   // the module's FSortSrc (where the sort writes its permutation, or null)
 
   hipGraph_t m_Graph = nullptr;
   hipGraphExec_t m_GraphExec = nullptr;
@@ -254,8 +256,11 @@ class Particles {
   void* m_AdjDrec = nullptr;             // 16 B x N pair records of the density adjoint
   void* m_AdjTabs = nullptr;             // FAccel tables routing ANBRBITS into the tape
   float* m_AdjVec = nullptr;             // gx, gv, ge, gF, gxs, ges (3N floats each), grho (N), target,
-                                         // target in the rollout frame (3N each)
+                                         // target in the rollout frame (3N each), a second gx, gv, ge
   uint32_t* m_AdjId0 = nullptr;          // original id of each slot at the rollout's start
+  uint32_t* m_AdjSrc = nullptr;          // the last sort's permutation (the module's FSortSrc points here
+                                         // during rollouts)
+  void** m_AdjSrcPtr = nullptr;          // device {m_AdjSrc, nullptr}: values copied into FSortSrc
   double* m_AdjPartial = nullptr;        // 3 per block
   double* m_AdjAccum = nullptr;          // 3 per scene
   void* m_AdjParams = nullptr;           // FParams_t per scene

@@ -45,6 +45,11 @@ __device__ __forceinline__ int nbr_cap() { return min(FNbrWords, FX_NBR_WORDS); 
 __constant__ int FHostCaps = 0;
 __device__ __forceinline__ bool host_tail() { return (FHostCaps & FX_CAP_TAIL) != 0; }
 __device__ __forceinline__ bool host_ranked() { return (FHostCaps & FX_CAP_RANKED) != 0; }
+
+// When set (the reverse mode's rollouts), the ranked counting sort writes
+// its permutation here: FSortSrc[s] = pre-sort index of the particle it puts
+// in slot s.  Null otherwise.
+__constant__ uint* FSortSrc = nullptr;
 #define FX_RANK_CAP 65536   // cells with more members keep atomic order
 
 // ---------------------------------------------------------------- helpers
@@ -684,6 +689,7 @@ __device__ __forceinline__ void sort_full_impl(const FParams_t& P, const Span S)
   if (has_id) pbuf<uint>(FPnts, FID)[s] = id;
   pbuf<int>(FPnts, FGCELL)[s] = c;
   pbuf<int>(FPnts, FGNDX)[s] = rank;
+  if (FSortSrc != nullptr) FSortSrc[s] = (uint)i;
 }
 
 // The counting sort of a host that runs no countingSortIndex (the reference's
