@@ -100,6 +100,13 @@ int launch_n(hipFunction_t f, const char* name, hipStream_t st, int n, void** ar
 }  // namespace
 
 void Particles::AdjRelease() {
+  if (m_AdjSrc != nullptr && m_SortSrcSym != nullptr) {
+    // a rollout that stopped on an error may have left the sort writing into
+    // the buffer about to be freed
+    void* null = nullptr;
+    (void)hipStreamSynchronize(m_Stream);
+    (void)hipMemcpyHtoD(m_SortSrcSym, &null, sizeof(null));
+  }
   for (void* p : m_AdjAlloc) (void)hipFree(p);
   m_AdjAlloc.clear();
   m_AdjCkpt = m_AdjTape = nullptr;
