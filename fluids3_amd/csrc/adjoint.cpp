@@ -121,7 +121,8 @@ void Particles::AdjRelease() {
   m_AdjPartial = m_AdjAccum = nullptr;
   m_AdjParams = m_AdjPBase = m_AdjBTab = m_AdjBFirst = nullptr;
   m_AdjBytes = 0;
-  m_AdjTapeValid = m_AdjLossDone = false;
+  m_AdjLossDone = false;
+  m_AdjBufSeg[0] = m_AdjBufSeg[1] = -1;
   m_AdjSteps = m_AdjEvery = m_AdjSegs = m_AdjN = m_AdjCells = 0;
 }
 
@@ -394,13 +395,19 @@ int Particles::AdjForward(double* loss, double* ms) {
     void* a[] = {&fid, &m_AdjId0, &tgt, &tgf, &nn};
     FX_TRY(launch_n(m_AdjFunc[ADJ_FRAME_IN], kAdjNames[ADJ_FRAME_IN], m_Stream, nn, a));
   }
+  // The last segment is recorded, and with two tape buffers the one before
+  // it too: the backward then finds both in the tape and recomputes only the
+  // segments before them.
   const int last = m_AdjSegs - 1;
+  const int first_rec = m_AdjBufs == 2 ? last - 1 : last;
+  m_AdjBufSeg[0] = m_AdjBufSeg[1] = -1;
   FX_TRY(AdjSortSrc(true));
   for (int t = 0; t < m_AdjSteps; t++) {
     const int seg = t / m_AdjEvery;
     if (t % m_AdjEvery == 0) FX_TRY(AdjSaveCheckpoint(seg));
-    FX_TRY(AdjStep(seg == last ? (seg % m_AdjBufs) * m_AdjTapeSteps + t - seg * m_AdjEvery : -1));
+    FX_TRY(AdjStep(seg >= first_rec ? (seg % m_AdjBufs) * m_AdjTapeSteps + t - seg * m_AdjEvery : -1));
   }
+  for (int sg = first_rec < 0 ? 0 : first_rec; sg <= last; sg++) m_AdjBufSeg[sg % m_AdjBufs] = sg;
   FX_TRY(AdjSortSrc(false));
   FX_TRY(AdjSaveCheckpoint(m_AdjSegs));
   // loss and seed on the final state
@@ -443,7 +450,6 @@ int Particles::AdjForward(double* loss, double* ms) {
     FX_HIP(hipEventElapsedTime(&f, e0.e, e1.e), "adjoint");
     *ms = f;
   }
-  m_AdjTapeValid = true;
   m_AdjLossDone = true;
   return FX_OK;
 }
@@ -457,8 +463,10 @@ int Particles::AdjRecompute(int seg) {
   FX_TRY(AdjRestoreCheckpoint(seg));
   const int s0 = seg * m_AdjEvery;
   const int s1 = (s0 + m_AdjEvery < m_AdjSteps) ? s0 + m_AdjEvery : m_AdjSteps;
+  m_AdjBufSeg[b] = -1;
   for (int t = s0; t < s1; t++) FX_TRY(AdjStep(b * m_AdjTapeSteps + t - s0));
   FX_HIP(hipEventRecord(m_AdjEv[b], m_Stream), "adjoint");
+  m_AdjBufSeg[b] = seg;
   return FX_OK;
 }
 
@@ -491,9 +499,11 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
   // step's outputs, the step writes that of its inputs into [cur ^ 1]
   float* set[2] = {V, V + 25 * n};
   int cur = 0;
-  if (m_AdjSegs > 0 && !m_AdjTapeValid) FX_TRY(AdjRecompute(m_AdjSegs - 1));
+  auto held = [&](int sg) { return m_AdjBufSeg[sg % m_AdjBufs] == sg; };
+  if (m_AdjSegs > 0 && !held(m_AdjSegs - 1)) FX_TRY(AdjRecompute(m_AdjSegs - 1));
   for (int seg = m_AdjSegs - 1; seg >= 0; seg--) {
-    if (seg > 0 && m_AdjBufs == 2) FX_TRY(AdjRecompute(seg - 1));  // beside the adjoint of this segment
+    // beside the adjoint of this segment, unless the forward recorded it
+    if (seg > 0 && m_AdjBufs == 2 && !held(seg - 1)) FX_TRY(AdjRecompute(seg - 1));
     const int b = seg % m_AdjBufs;
     FX_HIP(hipStreamWaitEvent(m_AdjStream, m_AdjEv[b], 0), "adjoint");
     const int s0 = seg * m_AdjEvery;
@@ -559,7 +569,6 @@ int Particles::AdjBackward(float* gpos, float* gvel, float* gveval, double* gvis
     void* b[] = {&g, &out, &m_AdjId0, &nn};
     FX_TRY(launch_n(m_AdjFunc[ADJ_UNFRAME], kAdjNames[ADJ_UNFRAME], m_Stream, nn, b));
   }
-  m_AdjTapeValid = m_AdjSegs <= 1;
   m_AdjLossDone = false;
   std::vector<double> acc(3 * (size_t)m_AdjScenes, 0.0);
   if (!first) {

@@ -242,7 +242,8 @@ class Particles {
   int m_AdjBufs = 1, m_AdjTapeSteps = 0;  // tape buffers (2 when segments are recomputed) and slots in each
   hipFunction_t m_AdjFunc[10] = {};
   int m_AdjSteps = 0, m_AdjEvery = 0, m_AdjSegs = 0, m_AdjN = 0, m_AdjCells = 0;
-  bool m_AdjTapeValid = false, m_AdjLossDone = false;
+  bool m_AdjLossDone = false;
+  int m_AdjBufSeg[2] = {-1, -1};  // the segment each tape buffer holds, -1 for none
   int64_t m_AdjBytes = 0;
   std::vector<void*> m_AdjAlloc;         // every device allocation of the adjoint
   std::vector<float> m_AdjCkptTime;      // checkpoint k = state before step k*every
