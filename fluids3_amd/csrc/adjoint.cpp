@@ -4,15 +4,19 @@
 //
 // A rollout of nsteps is cut into segments of `every` steps.  The forward
 // saves the state (pos, vel, veval, clr, id, time) before every segment and
-// after the last, and records the last segment's steps on the tape.  The
-// backward walks the segments in reverse: a segment whose steps are not on
-// the tape is recomputed from its checkpoint (the step is deterministic, so
-// the recomputed steps are bit-identical to the forward's), recording each
-// step right after its force stage; then the adjoint kernels run over the
+// after the last, and records the last segment's steps on the tape (the last
+// two segments' when the tape has two buffers).  The backward walks the
+// segments in reverse: a segment whose steps are not on the tape is
+// recomputed from its checkpoint (the step is deterministic, so the
+// recomputed steps are bit-identical to the forward's), recording each step
+// right after its force stage; then the adjoint kernels run over the
 // recorded steps in reverse.  The adjoint kernels run on a second stream, so
 // that segment s-1 is recomputed (into a second tape buffer) while the
-// adjoint of segment s runs.  For the length of a rollout the particles are
-// numbered by their slot at its start (see adjFrameIn).  Memory: (segments
+// adjoint of segment s runs.  The sorts of a rollout record their
+// permutations (FSortSrc), through which the adjoint state moves from one
+// step's slot order to the one before it.  For the length of a rollout the
+// particles are also numbered by their slot at its start (see adjFrameIn).
+// Memory: (segments
 // + 1) x 44 B x N of checkpoints plus every x (60 B x N + 8 B x cells +
 // W x 4 B x N of neighbour bitmask, W sized from the densest stencil) of tape, twice when there is
 // more than one segment and the second buffer fits in 60 % of the free
