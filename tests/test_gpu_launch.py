@@ -3,6 +3,7 @@
 - particles per thread 2, 4 and 8 for the streaming stages (insert, sort,
   advance), at two block sizes, on a particle count that leaves ragged last
   blocks, against one particle per thread;
+- the same with a third of the particles out of the grid (the tail bucket);
 - a batch of scenes run with persistent stage launches (a fixed grid pulling
   (scene, block) items from per-XCD queues), with one block per item, and
   with particles per thread 4, eagerly and as the captured graph.
@@ -55,6 +56,31 @@ def test_particles_per_thread_is_bit_identical():
         with pytest.raises(Exception):
             ps.SetStagePpt(abi.STAGE_NAMES.index("force"), 2)  # pair stages take one particle per lane
 
+
+
+def test_tail_bucket_at_every_launch_shape():
+    """A third of the particles leave the grid (the insert counts the tail
+    bucket once per block through LDS, blocks holding tail and in-range
+    particles alike): every block size and particles-per-thread count, and the
+    captured graph, give the same bits, and the bucket's count is exact."""
+    sc = scenes.dambreak_16k(n=10007)
+    with Particles() as ps:
+        ps.Restart(sc)
+        pos = ps.state()["pos"].copy()
+        pos[::4, 0] += 1000.0   # 2,502 particles beyond the grid in x
+        pos[1::16, 1] = -50.0   # 626 more below it
+        pos[2::64] = np.nan     # 157 more that are not a number
+
+        def restart():
+            ps.Restart(sc)
+            ps.Commit(abi.FPOS, pos)
+
+        ref = run_cfg(ps, restart, 256, 1, 4)
+        assert int(ref["cnt"][-1]) == 2502 + 626 + 157
+        for threads, ppt, graph in ((64, 1, False), (128, 2, False), (512, 8, False), (1024, 4, False),
+                                    (64, 4, True), (1024, 1, True)):
+            print(f"tail bucket: {threads} threads, ppt {ppt}, graph {graph}", flush=True)
+            same(run_cfg(ps, restart, threads, ppt, 4, graph), ref, (threads, ppt, graph))
 
 
 def test_batch_persistent_launch_is_bit_identical():
