@@ -472,10 +472,10 @@ int Particles::InsertParticles() {
 int Particles::PrefixScanParticles() {
   // particles.cpp:774-815, over gridTotal+1 counts; then the out-of-range
   // flags, whose exclusive scan ranks the tail bucket in particle order.
-  // Two launches per scan (the block totals go to the first auxiliary
-  // level, which holds one entry per 1024 elements); the reference's
-  // three-level prefixSum/prefixFixup beyond FX_SCAN2_MAX_BLOCKS blocks.
-  // both scans share the two launches
+  // Both scans in two launches (block totals first, into the first
+  // auxiliary level of each, which holds one entry per 1024 elements); the
+  // reference's three-level prefixSum/prefixFixup for a scan beyond
+  // FX_SCAN2_MAX_BLOCKS blocks.
   FX_TRY(ScanTwoLaunch(m_Accel.gpu(AGRIDCNT), m_Accel.gpu(AGRIDOFF), m_Accel.gpu(AAUXARRAY1), m_ScanCells,
                        m_Accel.gpu(ATAILFLAG), m_Accel.gpu(ATAILSCAN), m_Accel.gpu(ATAILAUX1), mNumPoints));
   m_Done = FX_STAGE_PREFIX;
