@@ -126,6 +126,10 @@ def roofline(stage_ms, N, G, C, P, batch=False):
     roof["kernel_ms"] = round(stage_ms[dom], 5)
     roof["algorithmic_bytes"] = by
     roof["algorithmic_flops"] = fl
+    if dom == "force":
+        # its two 16-byte record loads per pair, per second: read against the
+        # scattered-gather probe (tests/gather_rate), 1.03e12 from L1
+        roof["pair_loads_per_s"] = float(f"{2.0 * P * N / t_dom:.4g}")
     return roof
 
 
