@@ -821,8 +821,29 @@ __device__ __forceinline__ int stencil_base(const FParams_t& P, int gc) {
 // reads the part of the chunk that falls in its own row window.  Chunks are
 // visited in increasing order, so every lane still sees its candidates in the
 // reference's order.
+#ifndef FX_TILE_F3
+#define FX_TILE_F3 1  // tile entries of 3 floats (12-byte stride) instead of float4
+#endif
+// One tile entry (a candidate's position) in LDS: with a 12-byte stride the
+// entries of 21 consecutive candidates fall in distinct banks, so the lanes
+// of a wave that reads entries of several cells (rest density) do not queue
+// on a bank as entries 16 apart do with a 16-byte stride.
+struct TileP {
+#if FX_TILE_F3
+  float x, y, z;
+#else
+  float x, y, z, w;
+#endif
+};
+__device__ __forceinline__ void tile_put(TileP* t, int k, const fx_float3& q) {
+#if FX_TILE_F3
+  t[k] = TileP{q.x, q.y, q.z};
+#else
+  t[k] = TileP{q.x, q.y, q.z, 0.0f};
+#endif
+}
 #ifndef TILE_P
-#define TILE_P 1024  // float4 (pos, 0) per candidate, 16 KB
+#define TILE_P 1024  // candidates per tile (12 or 16 KB)
 #endif
 
 struct BlockSpan {
@@ -900,7 +921,7 @@ __device__ __forceinline__ int slab_start(const uint* __restrict__ sstart, int c
   return (n == 0 || k == 0) ? 0 : (k >= FX_SLABS ? n : (int)sstart[(size_t)c * FX_SLABS + k]);
 }
 
-__device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, int N, float4* tile, bool tail,
+__device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, int N, TileP* tile, bool tail,
                                               bool trim) {
   const int t = threadIdx.x;
   const int h0 = S.p0 + S.blk * (int)blockDim.x;
@@ -959,7 +980,7 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
       // highest) and the word is bit-reversed when stored.
       for (int k = t; k < hi - lo; k += blockDim.x) {
         const fx_float3 q = pos[lo + k];
-        tile[k] = make_float4(q.x, q.y, q.z, 0.0f);
+        tile_put(tile, k, q);
       }
       __syncthreads();
       const int a = act ? jb - lo : 0, len = act ? je - jb : 0;
@@ -971,7 +992,7 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
       auto cand = [&](int s, auto checked) {
         int k = a0 + s;
         if (decltype(checked)::value) k = min(k, last);
-        const float4 q = tile[k];
+        const TileP q = tile[k];
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         if (decltype(checked)::value) {
@@ -1038,12 +1059,12 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
       const int n = min(TILE_P, hi - c0);
       for (int k = t; k < n; k += blockDim.x) {
         const fx_float3 q = pos[c0 + k];
-        tile[k] = make_float4(q.x, q.y, q.z, 0.0f);
+        tile_put(tile, k, q);
       }
       __syncthreads();
       const int a = max(jb + ts, c0) - c0, b = min(jb + te, c0 + n) - c0;
       for (int k = a; k < b; k++) {
-        const float4 q = tile[k];
+        const TileP q = tile[k];
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         const bool pair = dsq < rd2 && dsq > 0.0f;
@@ -1082,19 +1103,19 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
-  __shared__ float4 tile[TILE_P];
+  __shared__ TileP tile[TILE_P];
   pressure_impl(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, tile, host_tail(), host_ranked());
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computePressureBatch(BatchArgs B, int ntotal) {
-  __shared__ float4 tile[TILE_P];
+  __shared__ TileP tile[TILE_P];
   const FParams_t* P;
   const Span S = batch_span(B, xcd_block(), P);
   pressure_impl(*P, S, ntotal, tile, true, true);
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computePressureBatchP(BatchArgs B, int ntotal) {
-  __shared__ float4 tile[TILE_P];
+  __shared__ TileP tile[TILE_P];
   batch_items(B, [&](int b) {
     const FParams_t* P;
     const Span S = batch_span(B, b, P);
