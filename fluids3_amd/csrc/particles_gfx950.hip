@@ -821,30 +821,25 @@ __device__ __forceinline__ int stencil_base(const FParams_t& P, int gc) {
 // reads the part of the chunk that falls in its own row window.  Chunks are
 // visited in increasing order, so every lane still sees its candidates in the
 // reference's order.
-#ifndef FX_TILE_F3
-#define FX_TILE_F3 1  // tile entries of 3 floats (12-byte stride) instead of float4
-#endif
-// One tile entry (a candidate's position) in LDS: with a 12-byte stride the
-// entries of 21 consecutive candidates fall in distinct banks, so the lanes
-// of a wave that reads entries of several cells (rest density) do not queue
-// on a bank as entries 16 apart do with a 16-byte stride.
+// One tile entry (a candidate's position) in LDS, 12 bytes: with a 16-byte
+// stride entries 16 apart share banks, and the lanes of a wave, whose windows
+// start at different entries (their x-range trims differ, and at rest density
+// they lie in different cells), queued on them; with a 12-byte stride the
+// entries of 21 consecutive candidates fall in distinct banks.  (Separate
+// x, y and z arrays measured the same as this.)
 struct TileP {
-#if FX_TILE_F3
   float x, y, z;
-#else
-  float x, y, z, w;
-#endif
 };
-__device__ __forceinline__ void tile_put(TileP* t, int k, const fx_float3& q) {
-#if FX_TILE_F3
-  t[k] = TileP{q.x, q.y, q.z};
-#else
-  t[k] = TileP{q.x, q.y, q.z, 0.0f};
-#endif
-}
 #ifndef TILE_P
-#define TILE_P 1024  // candidates per tile (12 or 16 KB)
+#define TILE_P 1024  // candidates per tile, 12 KB
 #endif
+#define FX_TILE_FLOATS (3 * TILE_P)
+__device__ __forceinline__ void tile_put(float* t, int k, const fx_float3& q) {
+  t[3 * k] = q.x;
+  t[3 * k + 1] = q.y;
+  t[3 * k + 2] = q.z;
+}
+__device__ __forceinline__ TileP tile_get(const float* t, int k) { return TileP{t[3 * k], t[3 * k + 1], t[3 * k + 2]}; }
 
 struct BlockSpan {
   int h1;      // end of the block's in-range particles
@@ -921,7 +916,7 @@ __device__ __forceinline__ int slab_start(const uint* __restrict__ sstart, int c
   return (n == 0 || k == 0) ? 0 : (k >= FX_SLABS ? n : (int)sstart[(size_t)c * FX_SLABS + k]);
 }
 
-__device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, int N, TileP* tile, bool tail,
+__device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, int N, float* tile, bool tail,
                                               bool trim) {
   const int t = threadIdx.x;
   const int h0 = S.p0 + S.blk * (int)blockDim.x;
@@ -992,7 +987,7 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
       auto cand = [&](int s, auto checked) {
         int k = a0 + s;
         if (decltype(checked)::value) k = min(k, last);
-        const TileP q = tile[k];
+        const TileP q = tile_get(tile, k);
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         if (decltype(checked)::value) {
@@ -1064,7 +1059,7 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
       __syncthreads();
       const int a = max(jb + ts, c0) - c0, b = min(jb + te, c0 + n) - c0;
       for (int k = a; k < b; k++) {
-        const TileP q = tile[k];
+        const TileP q = tile_get(tile, k);
         const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
         const float dsq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         const bool pair = dsq < rd2 && dsq > 0.0f;
@@ -1103,19 +1098,19 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computePressure(int pnum) {
-  __shared__ TileP tile[TILE_P];
+  __shared__ __align__(16) float tile[FX_TILE_FLOATS];
   pressure_impl(*cparams(&FParams), Span{0, pnum, 0, xcd_block()}, pnum, tile, host_tail(), host_ranked());
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computePressureBatch(BatchArgs B, int ntotal) {
-  __shared__ TileP tile[TILE_P];
+  __shared__ __align__(16) float tile[FX_TILE_FLOATS];
   const FParams_t* P;
   const Span S = batch_span(B, xcd_block(), P);
   pressure_impl(*P, S, ntotal, tile, true, true);
 }
 
 extern "C" __global__ __launch_bounds__(1024) void computePressureBatchP(BatchArgs B, int ntotal) {
-  __shared__ TileP tile[TILE_P];
+  __shared__ __align__(16) float tile[FX_TILE_FLOATS];
   batch_items(B, [&](int b) {
     const FParams_t* P;
     const Span S = batch_span(B, b, P);
