@@ -3,7 +3,8 @@
 - particles per thread 2, 4 and 8 for the streaming stages (insert, sort,
   advance), at two block sizes, on a particle count that leaves ragged last
   blocks, against one particle per thread;
-- the same with a third of the particles out of the grid (the tail bucket);
+- the same with a third of the particles out of the grid (the tail bucket), for
+  one scene and for a batch (one block per item and persistent);
 - a batch of scenes run with persistent stage launches (a fixed grid pulling
   (scene, block) items from per-XCD queues), with one block per item, and
   with particles per thread 4, eagerly and as the captured graph.
@@ -94,3 +95,29 @@ def test_batch_persistent_launch_is_bit_identical():
         for threads, ppt, graph in ((256, 1, False), (64, 4, False), (256, 4, True), (1024, 2, True)):
             print(f"persistent batch: {threads} threads, ppt {ppt}, graph {graph}", flush=True)
             same(run_cfg(ps, restart, threads, ppt, 6, graph), ref, (threads, ppt, graph))
+
+
+def test_batch_tail_buckets_at_every_launch_shape():
+    """Scenes of a batch with particles out of their grids (each scene's tail
+    bucket counted once per block): one block per item and persistent launches,
+    at several block sizes and particles per thread, eagerly and as the graph,
+    give the same bits."""
+    sc = scenes.batch_scenes(count=6, lo=800, hi=30000, seed=41)
+    with Particles() as ps:
+        ps.RestartBatch(sc)
+        pos = ps.state()["pos"].copy()
+        pos[::3, 0] += 1.0e4   # a third of every scene leaves its grid
+        pos[1::29, 1] = np.nan
+
+        def restart():
+            ps.RestartBatch(sc)
+            ps.Commit(abi.FPOS, pos)
+
+        ps.SetPersistent(False)
+        ref = run_cfg(ps, restart, 256, 1, 4)
+        for persistent, threads, ppt, graph in ((False, 64, 2, False), (False, 1024, 1, True), (True, 256, 1, False),
+                                                (True, 128, 4, True), (True, 1024, 2, False)):
+            ps.SetPersistent(persistent)
+            print(f"batch tails: persistent {persistent}, {threads} threads, ppt {ppt}, graph {graph}", flush=True)
+            same(run_cfg(ps, restart, threads, ppt, 4, graph), ref, (persistent, threads, ppt, graph))
+
