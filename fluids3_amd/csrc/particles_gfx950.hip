@@ -35,8 +35,9 @@ __constant__ cuDataX FAccel;
 
 // Words of neighbour bitmask a particle may record: FX_NBR_WORDS unless the
 // host lowers it while the bitmask buffer is a smaller one (a reverse-mode
-// tape slot).  A particle whose stencil rows need more records no bits, and
-// the force stage tests its candidates in place.
+// tape slot).  A particle whose stencil rows need more records only the
+// first ones, which the force stage does not read: it tests that particle's
+// candidates in place.
 __constant__ int FNbrWords = FX_NBR_WORDS;
 __device__ __forceinline__ int nbr_cap() { return min(FNbrWords, FX_NBR_WORDS); }
 
@@ -883,7 +884,9 @@ __device__ __forceinline__ int nbr_words(const FParams_t& P, int gc, const uint*
 // in-radius pairs; row r of particle i occupies words W_r .. W_r+ceil(len/32)-1
 // at [w * N + i], N the size of the whole particle buffer) and the particle's
 // (pos, rho), (veval, 0) record.  A particle whose candidates need more than
-// FX_NBR_WORDS words records no bits.
+// the cap's words (FNbrWords) records only its first ones, which the force
+// stage does not read: it finds the particle over the cap from its own row
+// lengths and tests its candidates in place.
 //
 // Row trimming (when the sort ordered every cell's members by x slab and
 // wrote where each slab begins, `trim`).  In grid units (the insert's
@@ -937,7 +940,14 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
   const int srch = P.gridSrch;
   const float rd2 = P.rd2, d2 = P.d2;
   const size_t NN = (size_t)N;
-  const bool rec = wbits && act && nbr_words(P, gc, off, cnt) <= nbr_cap();
+  // The bitmask's words are written up to the cap: a particle that needs
+  // more is found so by the force stage from its own row lengths, which then
+  // tests its candidates in place and reads none of them.
+  const bool rec = wbits && act;
+  const int cap = nbr_cap();
+  auto put = [&](int w, uint v) {
+    if (rec && w < cap) bits[(size_t)w * NN + i] = v;
+  };
   // x-distance beyond which fl(dx * dx) > rd2: dx > rlim > sqrt(rd2) (1 + 2^-21)
   const float rlim = __fsqrt_rn(rd2) * (1.0f + 0x1p-20f);
   const float Rg = rlim * P.gridDelta.x + FX_TRIM_MARGIN + (float)P.gridRes.x * 0x1p-20f;
@@ -950,16 +960,19 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
   int W = 0;  // first word of the current row
   for (int r = 0; r < srch * srch; r++) {
     const int adj = P.gridAdj[r * srch] - shift;
+    // a row ends where the cell after its last one begins (an in-range
+    // particle's row is followed by a cell of the same grid row)
     const int lo = (int)off[B.cf + adj];
-    const int hi = (int)(off[B.cl + adj + srch - 1] + cnt[B.cl + adj + srch - 1]);
+    const int hi = (int)off[B.cl + adj + srch];
     int jb = 0, je = 0;
     if (act) {
       jb = (int)off[gc + adj];
-      je = (int)(off[gc + adj + srch - 1] + cnt[gc + adj + srch - 1]);
+      je = (int)off[gc + adj + srch];
     }
-    // the lane walks candidates [ts, te) of its row (relative to jb)
+    // the lane walks candidates [ts, te) of its row (relative to jb); a row
+    // of at most FX_RANK_WAVE candidates has no cell with slab starts
     int ts = 0, te = je - jb;
-    if (trim_lane) {
+    if (trim_lane && je - jb > FX_RANK_WAVE) {
       const int c0 = gc + adj, c2 = gc + adj + srch - 1;
       const int n0 = (int)cnt[c0], n2 = (int)cnt[c2];
       // slab starts exist for the cells the sort ranked by wave
@@ -1031,15 +1044,15 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
         for (; u < nu; u++) cand(s0 + u, std::false_type());
         for (; u < n; u++) cand(s0 + u, std::true_type());
         const uint chunk = __builtin_bitreverse32(acc << (32 - n));
-        if (rec && wb < nw) bits[(size_t)(W + wb) * NN + i] = carry | (chunk << ph);
+        if (wb < nw) put(W + wb, carry | (chunk << ph));
         carry = ph ? chunk >> (32 - ph) : 0u;
         wb++;
         acc = 0u;
       }
       if (rec) {  // the words the walk did not reach hold no pair
-        for (int w = 0; w < w0; w++) bits[(size_t)(W + w) * NN + i] = 0u;
-        if (wb < nw) bits[(size_t)(W + wb) * NN + i] = carry;
-        for (int w = wb + 1; w < nw; w++) bits[(size_t)(W + w) * NN + i] = 0u;
+        for (int w = 0; w < w0; w++) put(W + w, 0u);
+        if (wb < nw) put(W + wb, carry);
+        for (int w = wb + 1; w < nw; w++) put(W + w, 0u);
       }
       __syncthreads();
       W += (je - jb + 31) >> 5;
@@ -1049,7 +1062,7 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
     uint acc = 0, bit = 1u << (ts & 31);
     int wcur = W + (ts >> 5);  // current word
     if (rec)
-      for (int w = W; w < wcur; w++) bits[(size_t)w * NN + i] = 0u;  // skipped candidates
+      for (int w = W; w < wcur; w++) put(w, 0u);  // skipped candidates
     for (int c0 = lo; c0 < hi; c0 += TILE_P) {
       const int n = min(TILE_P, hi - c0);
       for (int k = t; k < n; k += blockDim.x) {
@@ -1070,7 +1083,7 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
         if (rec) {
           acc |= pair ? bit : 0u;
           if (bit == 0x80000000u) {  // word complete
-            bits[(size_t)wcur * NN + i] = acc;
+            put(wcur, acc);
             acc = 0;
             wcur++;
           }
@@ -1079,9 +1092,9 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
       }
       __syncthreads();
     }
-    if (rec && bit != 1u) bits[(size_t)wcur++ * NN + i] = acc;  // the walk's partial last word
+    if (rec && bit != 1u) put(wcur++, acc);  // the walk's partial last word
     if (rec)
-      for (int w = wcur; w < W + ((je - jb + 31) >> 5); w++) bits[(size_t)w * NN + i] = 0u;
+      for (int w = wcur; w < W + ((je - jb + 31) >> 5); w++) put(w, 0u);
     W += (je - jb + 31) >> 5;
   }
   if (act) {
