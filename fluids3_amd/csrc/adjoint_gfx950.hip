@@ -450,13 +450,15 @@ __device__ __forceinline__ int wave_min_i(int v) {
   return v;
 }
 
+// Row r of the stencil at base: candidates [jb, je).  A row ends where the
+// cell after its last one begins (an in-range particle's row is followed by
+// a cell of the same grid row), so two loads, not three.
 __device__ __forceinline__ void row_of(const FParams_t& P, int base, int r, const uint* off, const uint* cnt,
                                        int& jb, int& je) {
   const int srch = P.gridSrch;
   const int c0 = base + P.gridAdj[r * srch];
-  const int c1 = c0 + srch - 1;
   jb = (int)off[c0];
-  je = (int)(off[c1] + cnt[c1]);
+  je = (int)off[c0 + srch];
 }
 
 __device__ __forceinline__ int words_of(const FParams_t& P, int base, const uint* off, const uint* cnt) {
@@ -544,7 +546,7 @@ __device__ __forceinline__ void ring_walk(const FParams_t& P, const AdjArgs& A, 
 #endif
 
 template <int SR, int K, class Op>
-__device__ __forceinline__ void queue_walk(const FParams_t& P, const AdjArgs& A, int i, int base, bool use,
+__device__ __forceinline__ bool queue_walk(const FParams_t& P, const AdjArgs& A, int i, int base, bool use,
                                            uint* lq, Op& op) {
   constexpr int R = SR * SR;
   const int t = threadIdx.x;
@@ -561,6 +563,15 @@ __device__ __forceinline__ void queue_walk(const FParams_t& P, const AdjArgs& A,
     nwords += (je - jb + 31) >> 5;
     E_[r] = nwords;
   }
+  // a lane whose rows need more words than a tape slot holds recorded none
+  // of them: it walks nothing here
+  const bool over = use && nwords > A.ncap;
+  if (over) {
+    nwords = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) E_[r] = G[r] = 0;
+  }
+  use = use && !over;
   uint* Qw = lq;
   uint* Qb = lq + K * 64;
   uint cur = 0u, nxt = 0u;
@@ -640,18 +651,20 @@ __device__ __forceinline__ void queue_walk(const FParams_t& P, const AdjArgs& A,
     }
   }
   drain(wave_max_i(qn));
+  return over;
 }
 
 // Stencil rows of 2 or 3 cells through the word queue, 4 through the ring.
+// Returns whether the lane's rows need more bitmask words than a tape slot
+// holds (then it recorded no bits, and nothing was walked for it).
 template <int K, class Op>
-__device__ __forceinline__ void pair_walk(const FParams_t& P, const AdjArgs& A, int i, int base, bool use,
+__device__ __forceinline__ bool pair_walk(const FParams_t& P, const AdjArgs& A, int i, int base, bool use,
                                           uint* lq, Op& op) {
-  if (P.gridSrch == 3)
-    queue_walk<3, K>(P, A, i, base, use, lq, op);
-  else if (P.gridSrch == 2)
-    queue_walk<2, K>(P, A, i, base, use, lq, op);
-  else
-    ring_walk(P, A, i, base, use, lq, op);
+  if (P.gridSrch == 3) return queue_walk<3, K>(P, A, i, base, use, lq, op);
+  if (P.gridSrch == 2) return queue_walk<2, K>(P, A, i, base, use, lq, op);
+  const bool over = use && words_of(P, base, A.off, A.cnt) > A.ncap;
+  ring_walk(P, A, i, base, use && !over, lq, op);
+  return over;
 }
 
 // ------------------------------------------------------------------ force
@@ -746,8 +759,10 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) AW_FORCE_ATTR adjForce
     F.idi = 1.0f / F.di;
     F.qi = F.di - F.rest;
   }
-  const bool ovf = act && words_of(P, base, A.off, A.cnt) > A.ncap;
-  if (ovf) {  // no recorded bits: test the candidates in place
+  // a particle without recorded bits tests its candidates in place, in the
+  // same candidate order (after the walk, which took none of its pairs)
+  const bool ovf = pair_walk<AW_QF>(P, A, i, base, act, fq, F);
+  if (ovf) {
     const float rd2 = P.rd2;
     for (int r = 0; r < P.gridSrch * P.gridSrch; r++) {
       int jb, je;
@@ -760,7 +775,6 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) AW_FORCE_ATTR adjForce
       }
     }
   }
-  pair_walk<AW_QF>(P, A, i, base, act && !ovf, fq, F);
   if (act) {
     const fx_float3 x0 = A.gxs[i], e0 = A.ges[i];
     A.gxs[i] = fx_float3{x0.x + F.gpx, x0.y + F.gpy, x0.z + F.gpz};
@@ -824,8 +838,8 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjDensity(AdjArgs A) 
     D.px = a.x; D.py = a.y; D.pz = a.z;
     D.gi = a.w;
   }
-  const bool ovf = act && words_of(P, base, A.off, A.cnt) > A.ncap;
-  if (ovf) {
+  const bool ovf = pair_walk<AW_QD>(P, A, i, base, act, fq, D);
+  if (ovf) {  // no recorded bits: the candidates in place
     for (int r = 0; r < P.gridSrch * P.gridSrch; r++) {
       int jb, je;
       row_of(P, base, r, A.off, A.cnt, jb, je);
@@ -837,7 +851,6 @@ extern "C" __global__ void __launch_bounds__(ADJ_THREADS) adjDensity(AdjArgs A) 
       }
     }
   }
-  pair_walk<AW_QD>(P, A, i, base, act && !ovf, fq, D);
   if (i >= S.p1) return;
   const uint src = A.id[i];
   const fx_float3 x0 = A.gxs[i];
