@@ -800,13 +800,17 @@ extern "C" __global__ void countingSortFullBatchP(BatchArgs B) {
 // (y*res.z + z)*res.x + x (particles.cpp:614-618) is contiguous in x, so the
 // srch cells of one (y, z) row are one contiguous run of sorted particles,
 // visited in the same order as the reference's per-cell loop.
+// Row r of the stencil at base (an in-range particle's): candidates
+// [jb, je).  The row ends where the cell after its last one begins -- a cell
+// of the same grid row, since an in-range particle's stencil stays one cell
+// inside the grid -- so two loads, not three.
 __device__ __forceinline__ void row_range(const FParams_t& P, int base, int r, const uint* __restrict__ off,
                                           const uint* __restrict__ cnt, int& jb, int& je) {
+  (void)cnt;
   const int srch = P.gridSrch;
   const int c0 = base + P.gridAdj[r * srch];
-  const int c1 = c0 + srch - 1;
   jb = (int)off[c0];
-  je = (int)(off[c1] + cnt[c1]);
+  je = (int)off[c0 + srch];
 }
 
 __device__ __forceinline__ int stencil_base(const FParams_t& P, int gc) {
