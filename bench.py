@@ -4,9 +4,9 @@
 Workload (default): the 4,000,000-particle dam break in the box
 (-120,0,-60)..(120,120,60) with the initial block filling the left 40 % of the
 box (0.40x rest spacing, about 15x rest density), FP32.  --warmup W untimed
-steps, then --steps K timed steps (the configuration states 20 + 200; with
-other values the line also carries the 20 + 200 window as
-"configured_window").  Inputs are resident in HBM before the timed region.
+steps, then --steps K timed steps (the configuration states 20 + 200, which
+is also what runs without flags; with other values the line also carries the
+20 + 200 window as "configured_window").  Inputs are resident in HBM before the timed region.
 
 Timed region: the K steps are captured as one HIP graph, with a HIP event
 before every stage and after the last, on the library's stream, and launched
@@ -15,8 +15,9 @@ give each kernel's mean duration, from which the roofline fraction of the
 dominant stage is computed with the algorithmic cost model:
   bytes per particle-step: insert 20, sort 92, density 20, force 44,
       advance 88 (264 in all), plus 16 per grid cell;
-  flops per particle-step: insert 6, density 8C+5P+2, force 8C+31P+2,
-      advance 60;
+  flops per particle-step: insert 6, density 8C+5P+2, force 31P+2 (the
+      candidate tests are done once, by the density stage, whose bitmask the
+      force stage walks), advance 60;
 C (stencil candidates) and P (in-radius pairs) per particle are measured on
 the same trajectory (the step is deterministic) every --stats-every steps.
 The roof of the pair stages is FP32 vector ALU issue ("valu", 157.3 TFLOP/s).
